@@ -1,0 +1,88 @@
+// Device-side data layout and kernels of the DSMC engine (gfx950).
+//
+// Particle state is structure-of-arrays in HBM, grouped by cell after every
+// step: positions in FP64, velocities and Ir in VT (double, or float for the
+// mixed-precision engine), cell (int32) and a stable id (uint32).  Two copies
+// of every array exist; the regroup gathers from one into the other.
+//
+// All kernels read the live particle count from device memory, so a whole
+// step (and a whole run) is enqueued without host round trips.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+#include "dsmc_detmath.h"
+#include "dsmc_rng.h"
+
+#pragma clang fp contract(off)
+
+namespace dsmc {
+
+// One triangle of a cell's face cache (FaceTriCache, src/ProcessDSMC.h:109-116):
+// vertex A, edges e1 = B - A, e2 = C - A, the face it belongs to and its tag
+// (-1 for triangular faces, 1/2 for the halves of a split quad).  80 bytes.
+struct TriRec {
+  double ax, ay, az, e1x, e1y, e1z, e2x, e2y, e2z;
+  int32_t face, tag;
+};
+
+enum FaceRole : int32_t { ROLE_NONE = 0, ROLE_INTERFACE = 1, ROLE_INLET = 2, ROLE_OUTLET = 3, ROLE_WALL = 4 };
+
+// c0/c1 = the two cells (c1 = -1 on boundaries), role, wall slot (-1 if not a wall).
+struct FaceRec {
+  int32_t c0, c1, role, wall;
+};
+
+// Inflow triangle patch (BoundaryEmitPatch + BoundaryGasState,
+// src/ProcessDSMC.h:157-205), gas state frozen at set-up.
+struct Patch {
+  double p0[3], e10[3], e20[3];
+  double R[9];  // rotation built from -n, row major
+  double stp1, stp2, snp, fs1, fs2, tN, sqrtT, QA;
+  double flux;  // inserted particles per step before the remainder (add_num / Neff)
+  int32_t cell, face, tag, pad;
+};
+
+// Live counters and statistics (device memory).
+struct Counters {
+  int32_t n;         // particles grouped by cell at step start
+  int32_t n_total;   // n + emitted this step
+  int32_t n_emit;    // emitted this step
+  int32_t overflow;  // capacity exceeded (sticky)
+  uint32_t next_id;
+  int32_t last_step;
+  int32_t log_count;
+  int32_t id_base;   // first id of this step's new inflow particles
+  unsigned long long emitted, removed, dropped, wall_hits, crossings, coll_pairs, collisions, particle_steps;
+};
+
+struct KParams {
+  uint64_t key;
+  double dtime, Tw, sqrtTw, Neff, dt_ref, coll_prefactor, v_rms, omega, alpha, P_relax, eta;
+  double inv_lb;  // 1 / (2.5 - omega)
+  double inv_alpha;
+  double dr, n_ref;
+  int32_t wall_model, rot, surface, NSS, NSCHEME, Nevery, Nrepeat, ncell;
+  int32_t capacity, npatch, ninlet_cells, max_iter;
+};
+
+struct CrossLog {
+  int32_t* ev;  // 6 ints per event
+  int32_t cap;
+  int32_t* count;
+};
+
+template <class VT>
+struct PBuf {
+  double *x, *y, *z;
+  VT *vx, *vy, *vz, *ir;
+  int32_t* cell;
+  uint32_t* id;
+};
+
+// Accumulator families (stepsum, steady, stepinter), structure of arrays
+// [family][field][cell], 18 fields each.
+constexpr int NMOM = 18;
+
+}  // namespace dsmc

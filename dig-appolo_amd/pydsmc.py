@@ -1,0 +1,379 @@
+"""Python binding of the MI355X DSMC engine's C ABI (include/dsmc_engine.h).
+
+Thin ctypes layer over ``libdsmc_engine.so`` (built in-tree next to this
+file).  Importing it without the built library raises: there is no CPU
+fallback for any engine call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdsmc_engine.so")
+
+DSMC_OK = 0
+STAGE_EMIT, STAGE_MOVE, STAGE_SORT, STAGE_COLLIDE, STAGE_SAMPLE = 1, 2, 4, 8, 16
+STAGE_ALL = 31
+MACRO_LOCAL, MACRO_RECORD, MACRO_FINAL, MACRO_OUTPUT = 0, 1, 2, 3
+WALL_DIFFUSE, WALL_SPECULAR = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+TAG_INTERFACE, TAG_INLET, TAG_OUTLET, TAG_WALL = 3, 4, 5, 6
+MACRO_NAMES = ["rho", "ux", "uy", "uz", "T", "sxx", "sxy", "syy", "syz", "szz", "szx",
+               "qx", "qy", "qz", "Tr", "qrx", "qry", "qrz"]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint32)
+_bp = C.POINTER(C.c_uint8)
+
+
+class MeshView(C.Structure):
+    _fields_ = [("ncell", C.c_int32), ("nface", C.c_int32), ("nnode", C.c_int32),
+                ("node_xyz", _dp), ("face_nodes", _ip), ("face_cells", _ip), ("face_type", _ip),
+                ("face_tag", _ip), ("face_split", _bp), ("face_normal", _dp), ("face_area", _dp),
+                ("cell_nface", _ip), ("cell_faces", _ip), ("cell_volume", _dp), ("cell_centroid", _dp)]
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("Ma", "Kn", "aoa_deg", "T_in", "T_wall", "T_ref", "L0", "gamma",
+                                          "omega", "alpha", "d_ref", "mass", "P_relax", "dr")] + \
+               [(n, C.c_int32) for n in ("Npinitial", "NSCHEME", "NSS", "Nevery", "Nrepeat", "wall_model",
+                                         "precision", "surface_stats", "inflow")] + \
+               [(n, C.c_double) for n in ("kB", "n_ref", "v_rms", "v_in", "mu_ref", "p_ref", "dt_ref",
+                                          "Twall_ref", "eta", "p_mass_r", "Neff", "dtime", "volume_total",
+                                          "min_edge")] + \
+               [("u_inf", C.c_double * 3), ("T_inf", C.c_double), ("coll_prefactor", C.c_double)]
+
+    def copy(self) -> "Params":
+        q = Params()
+        C.pointer(q)[0] = self
+        return q
+
+
+class ParticlesSoA(C.Structure):
+    _fields_ = [("n", C.c_int64), ("x", _dp), ("y", _dp), ("z", _dp), ("vx", _dp), ("vy", _dp), ("vz", _dp),
+                ("ir", _dp), ("cell", _ip), ("id", _up)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("particles", "emitted", "removed", "dropped", "wall_hits",
+                                         "crossings", "coll_pairs", "collisions", "particle_steps")] + \
+               [("last_step", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("calls", C.c_int64 * 8)]
+
+
+class CrossEvent(C.Structure):
+    _fields_ = [("id", C.c_uint32), ("step", C.c_int32), ("seq", C.c_int32), ("face", C.c_int32),
+                ("tri", C.c_int32), ("kind", C.c_int32)]
+
+
+KERNEL_GROUPS = ["emit", "move", "sort_count", "sort_scan", "sort_place", "sort_gather", "collide_sample", "other"]
+
+EXPORTED = [
+    "dsmc_mesh_build", "dsmc_mesh_box_tets", "dsmc_mesh_box_hexes", "dsmc_mesh_capsule", "dsmc_mesh_get_view",
+    "dsmc_mesh_destroy", "dsmc_params_apollo", "dsmc_params_box", "dsmc_params_derive", "dsmc_fill_freestream",
+    "dsmc_engine_create", "dsmc_engine_destroy", "dsmc_particles_upload", "dsmc_particles_download",
+    "dsmc_particle_count", "dsmc_step", "dsmc_run", "dsmc_track", "dsmc_track_log_enable", "dsmc_track_log_get",
+    "dsmc_macro_get", "dsmc_statistic_pre", "dsmc_accum_get", "dsmc_cell_scalars_get", "dsmc_surface_get",
+    "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
+    "dsmc_engine_footprint", "dsmc_device_math_check", "dsmc_last_error",
+]
+
+_lib = None
+
+
+def lib():
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"DSMC engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.dsmc_last_error.restype = C.c_char_p
+        for name in EXPORTED:
+            if name not in ("dsmc_last_error", "dsmc_mesh_destroy", "dsmc_engine_destroy"):
+                getattr(L, name).restype = C.c_int
+        L.dsmc_mesh_destroy.restype = None
+        L.dsmc_engine_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+class DsmcError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = ""):
+    if rc != DSMC_OK:
+        msg = lib().dsmc_last_error().decode(errors="replace")
+        raise DsmcError(f"{what}: error {rc}: {msg}")
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_math(u: np.ndarray) -> np.ndarray:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros((len(u), 6))
+    check(lib().dsmc_device_math_check(_ptr(u, C.c_double), C.c_int64(len(u)), _ptr(out, C.c_double)),
+          "device_math_check")
+    return out
+
+
+class Mesh:
+    """Host mesh owned by the library (dsmc_mesh)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        self.view = MeshView()
+        check(lib().dsmc_mesh_get_view(self.h, C.byref(self.view)), "mesh view")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().dsmc_mesh_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def box_tets(n: int, wall_tag: int = TAG_WALL) -> "Mesh":
+        h = C.c_void_p()
+        check(lib().dsmc_mesh_box_tets(n, wall_tag, C.byref(h)), "box_tets")
+        return Mesh(h)
+
+    @staticmethod
+    def box_hexes(n: int, wall_tag: int = TAG_WALL) -> "Mesh":
+        h = C.c_void_p()
+        check(lib().dsmc_mesh_box_hexes(n, wall_tag, C.byref(h)), "box_hexes")
+        return Mesh(h)
+
+    @staticmethod
+    def capsule(a: int = 25, c: int = 62, layers: int = 50, diameter: float = 3.912, tets: bool = False,
+                jitter: float = 0.0, seed: int = 0) -> "Mesh":
+        h = C.c_void_p()
+        check(lib().dsmc_mesh_capsule(a, c, layers, C.c_double(diameter), int(tets), C.c_double(jitter),
+                                      C.c_uint64(seed), C.byref(h)), "capsule")
+        return Mesh(h)
+
+    @staticmethod
+    def build(node_xyz, face_nodes, face_cells, face_type, face_tag, ncell: int) -> "Mesh":
+        node_xyz = np.ascontiguousarray(node_xyz, dtype=np.float64)
+        face_nodes = np.ascontiguousarray(face_nodes, dtype=np.int32)
+        face_cells = np.ascontiguousarray(face_cells, dtype=np.int32)
+        face_type = np.ascontiguousarray(face_type, dtype=np.int32)
+        face_tag = np.ascontiguousarray(face_tag, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().dsmc_mesh_build(ncell, len(face_type), len(node_xyz), _ptr(node_xyz, C.c_double),
+                                    _ptr(face_nodes, C.c_int32), _ptr(face_cells, C.c_int32),
+                                    _ptr(face_type, C.c_int32), _ptr(face_tag, C.c_int32), C.byref(h)),
+              "mesh_build")
+        return Mesh(h)
+
+    @property
+    def ncell(self):
+        return self.view.ncell
+
+    @property
+    def nface(self):
+        return self.view.nface
+
+    def arrays(self) -> dict:
+        v = self.view
+
+        def arr(p, n, dt):
+            return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+        return {
+            "node_xyz": arr(v.node_xyz, v.nnode * 3, np.float64).reshape(-1, 3),
+            "face_nodes": arr(v.face_nodes, v.nface * 4, np.int32).reshape(-1, 4),
+            "face_cells": arr(v.face_cells, v.nface * 2, np.int32).reshape(-1, 2),
+            "face_type": arr(v.face_type, v.nface, np.int32),
+            "face_tag": arr(v.face_tag, v.nface, np.int32),
+            "face_split": arr(v.face_split, v.nface, np.uint8),
+            "face_normal": arr(v.face_normal, v.nface * 3, np.float64).reshape(-1, 3),
+            "face_area": arr(v.face_area, v.nface, np.float64),
+            "cell_nface": arr(v.cell_nface, v.ncell, np.int32),
+            "cell_faces": arr(v.cell_faces, v.ncell * 6, np.int32).reshape(-1, 6),
+            "cell_volume": arr(v.cell_volume, v.ncell, np.float64),
+            "cell_centroid": arr(v.cell_centroid, v.ncell * 3, np.float64).reshape(-1, 3),
+        }
+
+
+def params_apollo(monatomic: bool = True) -> Params:
+    p = Params()
+    check(lib().dsmc_params_apollo(C.byref(p), int(monatomic)), "params_apollo")
+    return p
+
+
+def params_box(Kn: float = 0.05) -> Params:
+    p = Params()
+    check(lib().dsmc_params_box(C.byref(p), C.c_double(Kn)), "params_box")
+    return p
+
+
+def derive(mesh: Mesh, p: Params) -> Params:
+    check(lib().dsmc_params_derive(C.byref(mesh.view), C.byref(p)), "params_derive")
+    return p
+
+
+class Particles:
+    """Numpy-owned SoA particle set."""
+
+    FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ir")
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        for f in self.FIELDS:
+            setattr(self, f, np.zeros(self.n, dtype=np.float64))
+        self.cell = np.zeros(self.n, dtype=np.int32)
+        self.id = np.zeros(self.n, dtype=np.uint32)
+
+    def soa(self) -> ParticlesSoA:
+        s = ParticlesSoA()
+        s.n = self.n
+        for f in self.FIELDS:
+            setattr(s, f, _ptr(getattr(self, f), C.c_double))
+        s.cell = _ptr(self.cell, C.c_int32)
+        s.id = _ptr(self.id, C.c_uint32)
+        return s
+
+    def take(self, idx) -> "Particles":
+        q = Particles(len(idx))
+        for f in self.FIELDS + ("cell", "id"):
+            setattr(q, f, np.ascontiguousarray(getattr(self, f)[idx]))
+        return q
+
+    def copy(self) -> "Particles":
+        return self.take(np.arange(self.n))
+
+
+def fill_freestream(mesh: Mesh, p: Params, ppc: float, temperature: float = 0.0, seed: int = 1) -> Particles:
+    cnt = C.c_int64()
+    empty = ParticlesSoA()
+    check(lib().dsmc_fill_freestream(C.byref(mesh.view), C.byref(p), C.c_double(ppc), C.c_double(temperature),
+                                     C.c_uint64(seed), C.byref(empty), C.byref(cnt)), "fill count")
+    parts = Particles(cnt.value)
+    s = parts.soa()
+    check(lib().dsmc_fill_freestream(C.byref(mesh.view), C.byref(p), C.c_double(ppc), C.c_double(temperature),
+                                     C.c_uint64(seed), C.byref(s), C.byref(cnt)), "fill")
+    return parts
+
+
+class Engine:
+    """One DSMC case resident on the current HIP device."""
+
+    def __init__(self, mesh: Mesh, p: Params, seed: int = 1, capacity: int = 0):
+        self.mesh = mesh
+        self.params = p
+        self.h = C.c_void_p()
+        check(lib().dsmc_engine_create(C.byref(mesh.view), C.byref(p), C.c_uint64(seed), C.c_int64(capacity),
+                                       C.byref(self.h)), "engine_create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dsmc_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload(self, parts: Particles):
+        s = parts.soa()
+        check(lib().dsmc_particles_upload(self.h, C.byref(s)), "upload")
+
+    def count(self) -> int:
+        n = C.c_int64()
+        check(lib().dsmc_particle_count(self.h, C.byref(n)), "count")
+        return n.value
+
+    def download(self) -> Particles:
+        n = self.count()
+        parts = Particles(n)
+        s = parts.soa()
+        cnt = C.c_int64()
+        check(lib().dsmc_particles_download(self.h, C.byref(s), C.byref(cnt)), "download")
+        return parts
+
+    def step(self, istep: int, mask: int = STAGE_ALL):
+        check(lib().dsmc_step(self.h, istep, mask), f"step {istep}")
+
+    def run(self, first: int, last: int) -> float:
+        check(lib().dsmc_run(self.h, first, last), f"run {first}..{last}")
+        ms = C.c_double()
+        check(lib().dsmc_last_run_ms(self.h, C.byref(ms)), "last_run_ms")
+        return ms.value
+
+    def track(self, istep: int):
+        check(lib().dsmc_track(self.h, istep), "track")
+
+    def log_enable(self, max_events: int):
+        check(lib().dsmc_track_log_enable(self.h, C.c_int64(max_events)), "log_enable")
+
+    def log_get(self) -> np.ndarray:
+        n = C.c_int64()
+        check(lib().dsmc_track_log_get(self.h, None, C.c_int64(0), C.byref(n)), "log_count")
+        out = (CrossEvent * max(1, n.value))()
+        check(lib().dsmc_track_log_get(self.h, out, C.c_int64(n.value), C.byref(n)), "log_get")
+        a = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_int32)), shape=(max(1, n.value) * 6,))
+        return a.reshape(-1, 6)[: n.value].copy()
+
+    def macro(self, which: int = MACRO_LOCAL) -> np.ndarray:
+        out = np.zeros((self.mesh.ncell, 18))
+        check(lib().dsmc_macro_get(self.h, which, _ptr(out, C.c_double)), "macro_get")
+        return out
+
+    def statistic_pre(self):
+        check(lib().dsmc_statistic_pre(self.h), "statistic_pre")
+
+    def accum(self, family: int) -> np.ndarray:
+        out = np.zeros((self.mesh.ncell, 18))
+        check(lib().dsmc_accum_get(self.h, family, _ptr(out, C.c_double)), "accum_get")
+        return out
+
+    def cell_scalars(self) -> dict:
+        nc = self.mesh.ncell
+        d = {"count": np.zeros(nc, np.int32), "crmax": np.zeros(nc), "rem_coll": np.zeros(nc),
+             "rem_emit": np.zeros(nc), "sparse_state": np.zeros(nc, np.uint8), "sparse_steps": np.zeros(nc, np.int32)}
+        check(lib().dsmc_cell_scalars_get(self.h, _ptr(d["count"], C.c_int32), _ptr(d["crmax"], C.c_double),
+                                          _ptr(d["rem_coll"], C.c_double), _ptr(d["rem_emit"], C.c_double),
+                                          _ptr(d["sparse_state"], C.c_uint8), _ptr(d["sparse_steps"], C.c_int32)),
+              "cell_scalars")
+        return d
+
+    def surface(self) -> np.ndarray:
+        out = np.zeros((self.mesh.nface, 6))
+        check(lib().dsmc_surface_get(self.h, _ptr(out, C.c_double)), "surface_get")
+        return out
+
+    def surface_fields(self, istep: int, diameter: float):
+        f = np.zeros((self.mesh.nface, 12))
+        cd = C.c_double()
+        q = C.c_double()
+        check(lib().dsmc_surface_fields(self.h, istep, C.c_double(diameter), _ptr(f, C.c_double), C.byref(cd),
+                                        C.byref(q)), "surface_fields")
+        return f, cd.value, q.value
+
+    def stats(self) -> dict:
+        s = Stats()
+        check(lib().dsmc_stats_get(self.h, C.byref(s)), "stats")
+        return s.as_dict()
+
+    def profile(self, on: bool = True):
+        check(lib().dsmc_profile_enable(self.h, int(on)), "profile_enable")
+
+    def kernel_times(self) -> dict:
+        t = KernelTimes()
+        check(lib().dsmc_profile_get(self.h, C.byref(t)), "profile_get")
+        return {g: (t.ms[i], t.calls[i]) for i, g in enumerate(KERNEL_GROUPS)}
+
+    def footprint(self) -> int:
+        b = C.c_int64()
+        check(lib().dsmc_engine_footprint(self.h, C.byref(b)), "footprint")
+        return b.value

@@ -1,0 +1,327 @@
+/*
+ * dsmc_engine.h — C ABI of the MI355X DSMC particle engine.
+ *
+ * The engine replaces the per-step particle work of the reference's
+ * ProcessDSMC (src/ProcessDSMC.h:23-493): inflow emission, face-walk
+ * tracking with wall/inlet/outlet handling, regrouping particles by cell,
+ * NTC collisions with VHS scattering (and Larsen-Borgnakke rotational
+ * exchange), and the exponential/steady/interval cell sampling.  Everything
+ * stays resident in the GPU's HBM; the caller drives it once per time step in
+ * the stage order of the reference driver (src/meshImportTest.cpp:207-234).
+ *
+ * Conventions (same as the reference, src/meshImport.h:168-208):
+ *   - positions in mesh units; velocities in units of v_rms = sqrt(2 kB T_ref/m);
+ *     the move uses x += v * dtime (dtime in length units, 0.2 x min edge);
+ *   - cell "volume" is the reference's DsmcCell::area, face "area" its
+ *     DsmcEdge::length; face normals are unit vectors oriented from the
+ *     face's first cell c0 to its second cell c1 (outward on boundaries);
+ *   - macro outputs are ncell x 18 doubles in MacroIndex order
+ *     (src/ProcessDSMC.h:28-49): rho, ux, uy, uz, T, sxx, sxy, syy, syz, szz,
+ *     szx, qx, qy, qz, Tr, qrx, qry, qrz.
+ *
+ * Every function returns DSMC_OK (0) or a negative DSMC_ERR_* code; the text
+ * of the last error on the calling thread is returned by dsmc_last_error().
+ * The engine copies what it needs: no caller pointer is kept after a call
+ * returns.  One engine is not thread-safe; distinct engines may be used from
+ * distinct host threads.
+ */
+#ifndef DSMC_ENGINE_H
+#define DSMC_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DSMC_OK 0
+#define DSMC_ERR_ARG (-1)      /* bad argument / size                          */
+#define DSMC_ERR_HIP (-2)      /* HIP runtime failure (no GPU, launch error)   */
+#define DSMC_ERR_MESH (-3)     /* inconsistent mesh                            */
+#define DSMC_ERR_CAPACITY (-4) /* particle or log capacity exceeded            */
+#define DSMC_ERR_STATE (-5)    /* call out of order (e.g. no particles yet)    */
+
+/* ---- boundary roles (reference BoundaryRole, src/BoundaryCondition.h:10-24;
+ *      Fluent zone tags 3/4/5/6 map to interface/inlet/outlet/wall as in
+ *      BoundaryConditionTable::loadDefaults, src/BoundaryCondition.h:79-117) */
+#define DSMC_TAG_INTERFACE 3
+#define DSMC_TAG_INLET 4
+#define DSMC_TAG_OUTLET 5
+#define DSMC_TAG_WALL 6
+
+#define DSMC_WALL_DIFFUSE 0  /* reference model (src/ProcessDSMC.cpp:1793-1825) */
+#define DSMC_WALL_SPECULAR 1 /* declared but unimplemented in the reference
+                                (src/BoundaryCondition.h:44); new here        */
+
+#define DSMC_PREC_FP64 0
+#define DSMC_PREC_MIXED 1 /* FP32 velocities/Ir and collision arithmetic;
+                             FP64 positions, face tests, hit times, sums */
+
+/* Quad split tags (reference meshImport::FACE_SPLIT_*, src/meshImport.h:297-324) */
+#define DSMC_SPLIT_02 0
+#define DSMC_SPLIT_13 1
+#define DSMC_SPLIT_INVALID 255
+
+#define DSMC_MACRO_WIDTH 18
+#define DSMC_MAX_CELL_FACES 6
+
+/* ---- mesh ---------------------------------------------------------------- */
+/* Read-only view of an unstructured tet/hex/prism/pyramid mesh; the fields
+ * mirror DsmcCell / DsmcEdge / localPointXY / faceSplitTag
+ * (src/meshImport.h:168-189, src/ProcessDSMC.h:347-350) for a single rank
+ * that owns every cell. */
+typedef struct dsmc_mesh_view {
+  int32_t ncell, nface, nnode;
+  const double *node_xyz;      /* nnode*3                                         */
+  const int32_t *face_nodes;   /* nface*4: faceMap[0..3]; [3] = -1 for triangles  */
+  const int32_t *face_cells;   /* nface*2: faceMap[4..5] = c0, c1; c1 < 0 or >= ncell
+                                  marks a boundary face                            */
+  const int32_t *face_type;    /* nface: 3 (triangle) or 4 (quad)                  */
+  const int32_t *face_tag;     /* nface: zone tag 3/4/5/6                          */
+  const uint8_t *face_split;   /* nface: DSMC_SPLIT_* for quads                    */
+  const double *face_normal;   /* nface*3: unit, c0 -> c1                          */
+  const double *face_area;     /* nface                                            */
+  const int32_t *cell_nface;   /* ncell: DsmcCell::num                              */
+  const int32_t *cell_faces;   /* ncell*6: DsmcCell::cell2face, -1 padded          */
+  const double *cell_volume;   /* ncell: DsmcCell::area                             */
+  const double *cell_centroid; /* ncell*3: DsmcCell::cellXY                          */
+} dsmc_mesh_view;
+
+/* Host-side mesh object produced by the builders below (owns its arrays). */
+typedef struct dsmc_mesh dsmc_mesh;
+
+/* Build a mesh from raw connectivity: face nodes, face cells and tags.
+ * Computes face normals/areas/centroids and cell volumes/centroids with the
+ * reference's finite-volume formulas (setFaceCellNew + preprocess,
+ * src/meshImport.cpp:590-728,742-797), cell->face lists in face order, and
+ * the quad split tags (build_face_split_tags_from_faces,
+ * src/meshImportDsmcTopology.cpp:106-168). */
+int dsmc_mesh_build(int32_t ncell, int32_t nface, int32_t nnode, const double *node_xyz,
+                    const int32_t *face_nodes, const int32_t *face_cells,
+                    const int32_t *face_type, const int32_t *face_tag, dsmc_mesh **out);
+/* Unit cube [0,1]^3 of n^3 hexes, each split into 6 tets (Freudenthal), every
+ * boundary face tagged `wall_tag`. */
+int dsmc_mesh_box_tets(int32_t n, int32_t wall_tag, dsmc_mesh **out);
+/* Unit cube [0,1]^3 of n^3 hexes (quad faces), boundary tagged `wall_tag`. */
+int dsmc_mesh_box_hexes(int32_t n, int32_t wall_tag, dsmc_mesh **out);
+/* Capsule hex O-grid: a closed quad surface of box topology a x a x c on a
+ * spherical-cap heat shield + 33 degree cone after-body of diameter
+ * `diameter`, extruded in `layers` geometric layers to an outer ellipsoid.
+ * Outer faces whose centroid lies on the upstream half are inlet (tag 4), the
+ * rest outlet (tag 5); the body is wall (tag 6).  a=25, c=62, layers=50 gives
+ * the decoded Apollo topology (372,500 cells, 380,052 nodes, 1,124,950 faces).
+ * `tets` != 0 splits every hex into 12 tets around its centroid.  `jitter`
+ * (0..0.3) perturbs interior nodes, seeded by `seed`. */
+int dsmc_mesh_capsule(int32_t a, int32_t c, int32_t layers, double diameter, int32_t tets,
+                      double jitter, uint64_t seed, dsmc_mesh **out);
+int dsmc_mesh_get_view(const dsmc_mesh *m, dsmc_mesh_view *view);
+void dsmc_mesh_destroy(dsmc_mesh *m);
+
+/* ---- case parameters ----------------------------------------------------- */
+/* Inputs are the reference's compile-time constants lifted to run time
+ * (src/meshImport.h:45-67, src/meshImport.cpp:1275-1332, AoA from
+ * src/ProcessDSMC.cpp:2330-2343).  dsmc_params_derive fills the derived
+ * block exactly as setMeshMessage/setParticleMessage
+ * (src/meshImport.cpp:1273-1425) do. */
+typedef struct dsmc_params {
+  /* inputs */
+  double Ma, Kn, aoa_deg; /* free-stream Mach, Knudsen, angle of attack (x-y plane) */
+  double T_in, T_wall;    /* free-stream and wall temperatures [K]              */
+  double T_ref;           /* reference temperature [K] (273.15)                 */
+  double L0;              /* reference length [m]                               */
+  double gamma;           /* inflow speed uses Ma*sqrt(gamma/2)                 */
+  double omega, alpha;    /* VHS viscosity index, VSS alpha (1 = VHS)           */
+  double d_ref, mass;     /* reference diameter [m], molecular mass [kg]        */
+  double P_relax;         /* Larsen-Borgnakke probability (0 = monatomic)       */
+  double dr;              /* rotational DOF in outputs (<= 0: monatomic)        */
+  int32_t Npinitial;      /* particles per cell at set-up (Neff)                */
+  int32_t NSCHEME, NSS;   /* exponential window length, steady start step       */
+  int32_t Nevery, Nrepeat;/* interval sampling (NGSIS = Nevery*Nrepeat)         */
+  int32_t wall_model;     /* DSMC_WALL_*                                        */
+  int32_t precision;      /* DSMC_PREC_*                                        */
+  int32_t surface_stats;  /* 1: wall tallies after NSS (reference flag
+                             EnableBoundaryStressHeatStatistic)                 */
+  int32_t inflow;         /* 1: inject at inlet faces                           */
+  /* derived (dsmc_params_derive) */
+  double kB, n_ref, v_rms, v_in, mu_ref, p_ref, dt_ref, Twall_ref, eta, p_mass_r;
+  double Neff, dtime, volume_total, min_edge;
+  double u_inf[3];        /* inflow velocity in v_rms units                     */
+  double T_inf;           /* T_in / T_ref                                       */
+  double coll_prefactor;  /* pi d^2 (2 kB T_ref / m_r)^eta / Gamma(5/2-omega)  */
+} dsmc_params;
+
+/* Apollo case (src/meshImportTest.cpp:124, src/meshImport.cpp:1315-1332):
+ * Ma 5, Kn 0.01, AoA 30, T_in 142.2 K, T_w 300 K, L0 3.912 m, N2-like gas.
+ * monatomic != 0 selects the build's monatomic VHS mode (P_relax 0,
+ * gamma 5/3, dr 0). */
+int dsmc_params_apollo(dsmc_params *p, int32_t monatomic);
+/* Equilibrium box: hard spheres (omega 0.5), T = 273.15 K, L0 = 1 m,
+ * specular walls, no inflow. */
+int dsmc_params_box(dsmc_params *p, double Kn);
+int dsmc_params_derive(const dsmc_mesh_view *mesh, dsmc_params *p);
+
+/* ---- particles ----------------------------------------------------------- */
+/* Structure-of-arrays particle exchange record.  The reference's per-cell
+ * buckets (ParticleBucketSoA, src/MeshparticalInitial.h:40-186) flattened:
+ * `cell` replaces p_mesh_serial; `id` is a stable identity (the reference's
+ * p_serial is renumbered every step) used to key wall random draws. */
+typedef struct dsmc_particles_soa {
+  int64_t n;
+  double *x, *y, *z;
+  double *vx, *vy, *vz;
+  double *ir;
+  int32_t *cell;
+  uint32_t *id;
+} dsmc_particles_soa;
+
+/* Initial fill: Npc per cell proportional to volume with random rounding
+ * (src/meshImport.cpp:1409-1421), positions uniform over the cell's tet
+ * decomposition (src/MeshparticalInitial.cpp:347-361,481-721), a drifting
+ * Maxwellian at the free-stream state (u_inf, T_inf; T = `temperature` if
+ * > 0) and Ir ~ Exp(T/2) (0 when monatomic).  Call with soa->n = 0 and null
+ * arrays to get the count in *count; then with arrays of that size. */
+int dsmc_fill_freestream(const dsmc_mesh_view *mesh, const dsmc_params *p, double ppc,
+                         double temperature, uint64_t seed, dsmc_particles_soa *soa,
+                         int64_t *count);
+
+/* ---- engine --------------------------------------------------------------- */
+typedef struct dsmc_engine dsmc_engine;
+
+/* Stage bits for dsmc_step, in the reference driver's order. */
+#define DSMC_STAGE_EMIT 1u    /* preprocesseffquad  (src/ProcessDSMC.cpp:2556)  */
+#define DSMC_STAGE_MOVE 2u    /* advection          (src/ProcessDSMC.cpp:1946)  */
+#define DSMC_STAGE_SORT 4u    /* cache2chain        (src/ProcessDSMC.cpp:2030)  */
+#define DSMC_STAGE_COLLIDE 8u /* collisionDSMC      (src/ProcessDSMC.cpp:2621)  */
+#define DSMC_STAGE_SAMPLE 16u /* statistic_macro    (src/ProcessDSMC.cpp:2757)  */
+#define DSMC_STAGE_ALL 31u
+
+/* Create an engine on the current HIP device: copies the mesh, builds the
+ * per-cell triangle cache (rebuildFaceTraversalCache,
+ * src/ProcessDSMCGeometry.cpp:145-214), the face crossing table
+ * (src/ProcessDSMCGeometry.cpp:114-136), wall normals (processFacesQuadNormals,
+ * src/ProcessDSMC.cpp:1054-1128) and inflow patches (rebuildBoundaryEmitCache,
+ * src/ProcessDSMC.cpp:583-768).  `capacity` bounds the particle count
+ * (0: 1.5 x Npinitial x ncell + inflow headroom).  `p` must be derived. */
+int dsmc_engine_create(const dsmc_mesh_view *mesh, const dsmc_params *p, uint64_t seed,
+                       int64_t capacity, dsmc_engine **out);
+void dsmc_engine_destroy(dsmc_engine *e);
+
+/* Copy particles in (any order; they are regrouped by cell, keeping their
+ * order inside a cell) / out (cell order).  Upload resets per-cell collision
+ * state (crmax = 3.0, remainders 0 — Collision_constant_initial,
+ * src/MeshparticalInitial.cpp:798-810) and all accumulators.  Download with
+ * soa->n smaller than the particle count returns DSMC_ERR_CAPACITY and the
+ * needed size in *count. */
+int dsmc_particles_upload(dsmc_engine *e, const dsmc_particles_soa *soa);
+int dsmc_particles_download(dsmc_engine *e, dsmc_particles_soa *soa, int64_t *count);
+int dsmc_particle_count(dsmc_engine *e, int64_t *count);
+
+/* One time step: the stages in `stage_mask`, in the order emit, move, sort,
+ * collide, sample.  A stage that needs the particles grouped by cell (emit
+ * appends, collide/sample read cells) requires the previous stages to have
+ * run in order; DSMC_STAGE_ALL is one full reference step. */
+int dsmc_step(dsmc_engine *e, int32_t istep, uint32_t stage_mask);
+/* Run steps first..last (inclusive) with DSMC_STAGE_ALL, asynchronously on
+ * the engine's stream; synchronises at the end. */
+int dsmc_run(dsmc_engine *e, int32_t first, int32_t last);
+/* Move only (the minimum slice): advection of every particle for one dtime
+ * (src/ProcessDSMC.cpp:1946-2017) followed by the regroup. */
+int dsmc_track(dsmc_engine *e, int32_t istep);
+
+/* Crossing log for tracking parity: when enabled, every face crossing or wall
+ * hit appends (id, step, sequence#, face, triangle tag, kind).  kind: 1
+ * interface, 2 inlet/outlet exit, 3 wall, 4 stop inside cell, 5 dropped. */
+typedef struct dsmc_cross_event {
+  uint32_t id;
+  int32_t step;
+  int32_t seq;
+  int32_t face;
+  int32_t tri;
+  int32_t kind;
+} dsmc_cross_event;
+int dsmc_track_log_enable(dsmc_engine *e, int64_t max_events);
+int dsmc_track_log_get(dsmc_engine *e, dsmc_cross_event *out, int64_t cap, int64_t *count);
+
+/* Macro fields, ncell*18 doubles (MacroIndex order).  which:
+ *   DSMC_MACRO_LOCAL  exponential window  ("local",        src/ProcessDSMC.cpp:2792)
+ *   DSMC_MACRO_RECORD interval record     ("record",       :2849-2856)
+ *   DSMC_MACRO_FINAL  steady average      ("final_record", :2818-2819)
+ *   DSMC_MACRO_OUTPUT what out2dat writes (record before NSS or for sparse
+ *                     cells, else final_record, :3357-3368) */
+#define DSMC_MACRO_LOCAL 0
+#define DSMC_MACRO_RECORD 1
+#define DSMC_MACRO_FINAL 2
+#define DSMC_MACRO_OUTPUT 3
+int dsmc_macro_get(dsmc_engine *e, int32_t which, double *out);
+/* statistic_macroPre (src/ProcessDSMC.cpp:3312-3326): local and record from
+ * the current particles. */
+int dsmc_statistic_pre(dsmc_engine *e);
+/* Raw per-cell accumulators: family 0 stepsum, 1 steady, 2 stepinter;
+ * ncell*18 doubles in the order rho(N), Sum v (3), Sum|v|^2, Sum v_i v_j
+ * (xx,xy,yy,yz,zz,zx), Sum v_i|v|^2 (3), Sum Ir, Sum v_i Ir (3). */
+int dsmc_accum_get(dsmc_engine *e, int32_t family, double *out);
+/* Per-cell scalars, ncell each (null pointers skipped): particle count,
+ * crmax, collision remainder, inflow remainder, sparse-cell state,
+ * sparse accumulation steps. */
+int dsmc_cell_scalars_get(dsmc_engine *e, int32_t *count, double *crmax, double *rem_coll,
+                          double *rem_emit, uint8_t *sparse_state, int32_t *sparse_steps);
+
+/* Wall tallies, nface*6 doubles: hits, momentum change (3), translational and
+ * rotational energy change (src/ProcessDSMC.cpp:1659-1689); zero for
+ * non-wall faces. */
+int dsmc_surface_get(dsmc_engine *e, double *out);
+/* Wall output (outBoundaryStressHeat, src/ProcessDSMC.cpp:3494-3557) from
+ * the tallies at step `istep`: nface*12 doubles, per wall face stress(3),
+ * q_trans, q_rot, normal(3), pressure, shear(3) (zero for other faces);
+ * plus the drag coefficient (force on the wall along the free-stream
+ * direction over q_inf * pi (D/2)^2) and the peak of q_trans + q_rot. */
+#define DSMC_SURF_WIDTH 12
+int dsmc_surface_fields(dsmc_engine *e, int32_t istep, double ref_diameter, double *fields,
+                        double *drag_coefficient, double *peak_heat_flux);
+
+typedef struct dsmc_stats {
+  int64_t particles;     /* current count                                   */
+  int64_t emitted;       /* cumulative inflow particles                     */
+  int64_t removed;       /* cumulative inlet/outlet exits                   */
+  int64_t dropped;       /* cumulative tracking failures (no exit face)     */
+  int64_t wall_hits;     /* cumulative wall reflections                     */
+  int64_t crossings;     /* cumulative interface crossings                  */
+  int64_t coll_pairs;    /* cumulative NTC candidate pairs                  */
+  int64_t collisions;    /* cumulative accepted collisions                  */
+  int64_t particle_steps;/* sum over steps of the count at step start       */
+  int32_t last_step;
+  int32_t pad;
+} dsmc_stats;
+int dsmc_stats_get(dsmc_engine *e, dsmc_stats *s);
+
+/* Per-kernel timing with HIP events on the engine's stream. */
+#define DSMC_NKERNEL 8
+typedef struct dsmc_kernel_times {
+  double ms[DSMC_NKERNEL];    /* accumulated milliseconds per kernel group */
+  int64_t calls[DSMC_NKERNEL];
+} dsmc_kernel_times; /* groups: 0 emit, 1 move, 2 sort-count, 3 sort-scan,
+                        4 sort-place, 5 sort-gather, 6 collide+sample, 7 other */
+int dsmc_profile_enable(dsmc_engine *e, int32_t on);
+int dsmc_profile_get(dsmc_engine *e, dsmc_kernel_times *t);
+/* Wall-clock (HIP events around the whole step sequence) of the last
+ * dsmc_run call, milliseconds. */
+int dsmc_last_run_ms(dsmc_engine *e, double *ms);
+
+/* Bytes of device memory the engine holds. */
+int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
+
+/* Diagnostic: evaluates on the device, for each input u in (0, 1], the
+ * shared deterministic functions the kernels use — out[6i..6i+5] =
+ * log(u), exp(-20u), sin(2 pi u), cos(2 pi u), sqrt(u), u^0.26.  Lets a test
+ * check that device and host arithmetic agree bit for bit.  No counterpart
+ * in the reference. */
+int dsmc_device_math_check(const double *u, int64_t n, double *out);
+
+const char *dsmc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DSMC_ENGINE_H */
