@@ -1,0 +1,207 @@
+"""Benchmark: DSMC particle-steps/s on one MI355X.
+
+Workload (default): the equilibrium box — unit cube of 64^3 hexes x 6 tets
+(1,572,864 cells), hard-sphere gas at T = 273.15 K, Kn = 0.05, 50 particles
+per cell (~78.6M particles), specular walls, FP64.  One step = emit + move +
+regroup + collide + sample.  Inputs are resident in HBM before the timed
+region; the timed region is exactly --steps steps on the engine's stream,
+bracketed by device synchronisation.
+
+Also reported on the same JSON line: the mixed-precision engine on the same
+particles, the roofline of the dominant kernel (HIP-event timed over the same
+steps) and the CPU baseline (the C++ oracle on a bounded sample of the same
+workload, on the host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dig-appolo_amd"))
+
+import numpy as np  # noqa: E402
+
+import pydsmc as D  # noqa: E402
+
+METRIC = "particle-steps/s (move+collide+sample), FP64 vs mixed; p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per particle for one launch of each kernel group
+# (state S = 64 B FP64 / 48 B mixed, velocity+Ir S_v = 32 / 16 B,
+# triangle cache M_tri = 80 B x triangles per cell, 896 B per-cell sampling)
+
+
+def alg_bytes(kernel: str, S: int, Sv: int, mtri: float, ppc: float) -> float:
+    if kernel == "move":
+        return 2 * S + mtri / ppc
+    if kernel == "sort":
+        return 2 * S + 4
+    if kernel == "collide_sample":
+        return 2 * Sv + 896.0 / ppc
+    raise KeyError(kernel)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_case(n: int, ppc: float, mixed: bool):
+    mesh = D.Mesh.box_tets(n)
+    p = D.params_box(0.05)
+    p.Npinitial = int(round(ppc))
+    p.NSS = 1000
+    p.precision = D.PREC_MIXED if mixed else D.PREC_FP64
+    D.derive(mesh, p)
+    return mesh, p
+
+
+def run_engine(mesh, p, parts, steps, warmup, seed=20261203):
+    eng = D.Engine(mesh, p, seed=seed)
+    eng.upload(parts)
+    del_ms = 0.0
+    if warmup > 0:
+        eng.run(1, warmup)
+    st0 = eng.stats()
+    eng.profile(True)
+    t0 = time.perf_counter()
+    ms_events = eng.run(warmup + 1, warmup + steps)
+    wall = time.perf_counter() - t0
+    st1 = eng.stats()
+    kt = eng.kernel_times()
+    psteps = st1["particle_steps"] - st0["particle_steps"]
+    res = {
+        "particle_steps": psteps,
+        "event_ms": ms_events,
+        "wall_s": wall,
+        "value": psteps / (ms_events / 1e3),
+        "ms_per_step": ms_events / steps,
+        "kernels": kt,
+        "stats": st1,
+        "footprint_GB": eng.footprint() / 1e9,
+    }
+    eng.close()
+    del del_ms
+    return res
+
+
+def cpu_baseline(threads: int, seconds_target: float = 15.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+
+    n = 24
+    mesh, p = make_case(n, 50.0, False)
+    parts = D.fill_freestream(mesh, p, ppc=50.0, temperature=1.0, seed=7)
+    O.set_threads(threads)
+    orc = O.Oracle(mesh, p, seed=7)
+    orc.set_particles(parts)
+    orc.step(1)  # warm caches
+    st0 = orc.stats()["particle_steps"]
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        steps += 1
+        orc.step(1 + steps)
+        if time.perf_counter() - t0 > seconds_target or steps >= 50:
+            break
+    dt = time.perf_counter() - t0
+    ps = orc.stats()["particle_steps"] - st0
+    return {
+        "value": ps / dt,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"C++ oracle, {n}^3x6-tet box ({mesh.ncell} cells), 50 ppc ({parts.n} particles), "
+                  f"{steps} full steps in {dt:.1f} s, {threads} threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="box resolution (hexes per side)")
+    ap.add_argument("--ppc", type=float, default=50.0)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs on one GPU per case; multi-GPU runs are not part of this build "
+              "(use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    D.lib()  # fail loudly if the engine library is missing
+    t0 = time.perf_counter()
+    mesh, p = make_case(args.n, args.ppc, False)
+    parts = D.fill_freestream(mesh, p, ppc=args.ppc, temperature=1.0, seed=20261203)
+    log(f"[bench] box {args.n}^3x6 tets: {mesh.ncell} cells, {parts.n} particles "
+        f"(set-up {time.perf_counter() - t0:.1f} s)")
+    fp64 = run_engine(mesh, p, parts, args.steps, args.warmup)
+    log(f"[bench] fp64: {fp64['value']:.4g} particle-steps/s, {fp64['ms_per_step']:.2f} ms/step")
+
+    mixed = None
+    if not args.no_mixed:
+        pm = p.copy()
+        pm.precision = D.PREC_MIXED
+        mixed = run_engine(mesh, pm, parts, args.steps, args.warmup)
+        log(f"[bench] mixed: {mixed['value']:.4g} particle-steps/s, {mixed['ms_per_step']:.2f} ms/step")
+
+    # roofline of the dominant kernel group, from the HIP events around each
+    # launch over the timed steps
+    kt = fp64["kernels"]
+    groups = {"move": kt["move"][0], "sort": sum(kt[g][0] for g in ("sort_count", "sort_scan", "sort_place",
+                                                                   "sort_gather")),
+              "collide_sample": kt["collide_sample"][0]}
+    dom = max(groups, key=groups.get)
+    ntri = 4.0  # triangles per tet
+    per_particle = alg_bytes(dom, 64, 32, 80.0 * ntri, args.ppc)
+    nmean = fp64["particle_steps"] / args.steps
+    launch_ms = groups[dom] / args.steps
+    achieved = per_particle * nmean / (launch_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "bytes_per_particle": per_particle, "launch_ms": round(launch_ms, 4)}
+    step_bytes = sum(alg_bytes(k, 64, 32, 80.0 * ntri, args.ppc) for k in ("move", "sort", "collide_sample"))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(threads)
+        log(f"[bench] cpu baseline: {cpu['value']:.4g} particle-steps/s on {threads} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(fp64["value"], 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fp64["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Maxwellian fill of a generated mesh)",
+        "config": {"workload": f"equilibrium box {args.n}^3 hexes x 6 tets, hard spheres, Kn 0.05, "
+                               f"{args.ppc:g} particles/cell, specular walls, full step "
+                               "(emit+move+sort+collide+sample)",
+                   "cells": mesh.ncell, "particles": parts.n, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "step_alg_GBs": round(step_bytes * nmean / (fp64["ms_per_step"] / 1e3) / 1e9, 1),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]},
+        "mixed": None if mixed is None else {"value": round(mixed["value"], 1), "dtype": "f32 velocities / f64 positions",
+                                             "ms_per_step": round(mixed["ms_per_step"], 4),
+                                             "speedup_vs_fp64": round(mixed["value"] / fp64["value"], 3)},
+        "gpu_vs_cpu": None if cpu is None else round(fp64["value"] / cpu["value"], 1),
+        "footprint_GB": round(fp64["footprint_GB"], 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,321 @@
+"""Oracle physics against conservation laws and closed forms (CPU only)."""
+import math
+import re
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import pydsmc as D
+
+KB = 1.3806e-23
+
+
+def _box(n=4, ppc=30.0, Kn=0.05, wall=D.WALL_SPECULAR, T=1.0, seed=5, hex_=False):
+    m = D.Mesh.box_hexes(n) if hex_ else D.Mesh.box_tets(n)
+    p = D.params_box(Kn)
+    p.Npinitial = int(ppc)
+    p.wall_model = wall
+    D.derive(m, p)
+    parts = D.fill_freestream(m, p, ppc=ppc, temperature=T, seed=seed)
+    return m, p, parts
+
+
+def test_collisions_conserve_momentum_and_energy():
+    m, p, parts = _box(n=3, ppc=40)
+    p.P_relax, p.dr = 0.3, 2.0  # exercise Larsen-Borgnakke exchange
+    parts.ir[:] = np.random.default_rng(0).exponential(0.5, parts.n)
+    o = O.Oracle(m, p, seed=3)
+    o.set_particles(parts)
+    P0 = np.array([parts.vx.sum(), parts.vy.sum(), parts.vz.sum()])
+    E0 = 0.5 * (parts.vx ** 2 + parts.vy ** 2 + parts.vz ** 2).sum() + parts.ir.sum()
+    for s in range(1, 11):
+        o.step(s, D.STAGE_COLLIDE)
+    q = o.particles()
+    assert o.stats()["collisions"] > 1000
+    P1 = np.array([q.vx.sum(), q.vy.sum(), q.vz.sum()])
+    E1 = 0.5 * (q.vx ** 2 + q.vy ** 2 + q.vz ** 2).sum() + q.ir.sum()
+    assert np.abs(P1 - P0).max() < 1e-9
+    assert abs(E1 - E0) / E0 < 1e-13
+    assert not np.allclose(q.vx, parts.vx)  # something happened
+
+
+def test_specular_box_conserves_energy_and_count():
+    m, p, parts = _box(n=4, ppc=20)
+    o = O.Oracle(m, p, seed=7)
+    o.set_particles(parts)
+    E0 = (parts.vx ** 2 + parts.vy ** 2 + parts.vz ** 2).sum()
+    for s in range(1, 31):
+        o.step(s)
+    q = o.particles()
+    st = o.stats()
+    assert q.n == parts.n and st["dropped"] == 0 and st["wall_hits"] > 0
+    assert abs((q.vx ** 2 + q.vy ** 2 + q.vz ** 2).sum() - E0) / E0 < 1e-12
+    assert q.x.min() >= 0 and q.x.max() <= 1 and q.y.min() >= 0 and q.z.max() <= 1
+
+
+def _point_in_tet(P, tet):
+    a, b, c, d = tet
+    M = np.stack([b - a, c - a, d - a], axis=1)
+    lam = np.linalg.solve(M, P - a)
+    return np.all(lam >= -1e-9) and lam.sum() <= 1 + 1e-9
+
+
+def test_tracking_lands_in_the_right_cell():
+    m, p, parts = _box(n=3, ppc=15)
+    a = m.arrays()
+    o = O.Oracle(m, p, seed=1)
+    o.set_particles(parts)
+    o.step(1, D.STAGE_MOVE | D.STAGE_SORT)
+    q = o.particles()
+    # speeds are unchanged by specular walls; every particle is inside its cell
+    sp0 = np.sort(parts.vx ** 2 + parts.vy ** 2 + parts.vz ** 2)
+    sp1 = np.sort(q.vx ** 2 + q.vy ** 2 + q.vz ** 2)
+    np.testing.assert_allclose(sp0, sp1, rtol=1e-14)
+    nodes = a["node_xyz"]
+    for k in range(0, q.n, 37):
+        c = q.cell[k]
+        ids = np.unique(a["face_nodes"][a["cell_faces"][c][:4]][:, :3])
+        assert len(ids) == 4
+        assert _point_in_tet(np.array([q.x[k], q.y[k], q.z[k]]), nodes[ids])
+    # free flight: particles that hit no wall moved by exactly v * dtime
+    ev = o.log_get() if False else None  # log not enabled here
+    byid0 = np.argsort(parts.id)
+    byid1 = np.argsort(q.id)
+    x0 = np.stack([parts.x, parts.y, parts.z], 1)[byid0]
+    x1 = np.stack([q.x, q.y, q.z], 1)[byid1]
+    v0 = np.stack([parts.vx, parts.vy, parts.vz], 1)[byid0]
+    v1 = np.stack([q.vx, q.vy, q.vz], 1)[byid1]
+    free = np.all(v0 == v1, axis=1)
+    assert free.mean() > 0.5
+    np.testing.assert_allclose(x1[free], x0[free] + v0[free] * p.dtime, atol=1e-13)
+
+
+def test_hard_sphere_collision_rate():
+    # Bird (1994) eq. 1.10 / 4.64: nu = n sigma <c_r>, <c_r> = 4 sqrt(kT/(pi m));
+    # collisions per cell and step = 1/2 N (N-1)/V Neff sigma <c_r> dt.
+    m, p, parts = _box(n=4, ppc=40, Kn=0.05)
+    o = O.Oracle(m, p, seed=9)
+    o.set_particles(parts)
+    steps = 8
+    counts = np.bincount(parts.cell, minlength=m.ncell).astype(float)
+    vol = m.arrays()["cell_volume"]
+    sigma = math.pi * p.d_ref ** 2
+    cr = 4 * math.sqrt(KB * p.T_ref / (math.pi * p.mass))
+    dt = p.dtime * p.dt_ref
+    per_step = np.sum(0.5 * counts * (counts - 1) / vol) * p.Neff * sigma * cr * dt
+    for s in range(1, steps + 1):
+        o.step(s, D.STAGE_COLLIDE)
+    got = o.stats()["collisions"]
+    want = per_step * steps
+    assert abs(got - want) < 4 * math.sqrt(want) + 0.02 * want, (got, want)
+
+
+def test_diffuse_wall_half_range_maxwellian():
+    # one hex cell, particles launched into the x=0 wall: the re-emitted
+    # normal speed has mean sqrt(pi Tw)/2, tangential components variance
+    # Tw/2, rotational energy mean Tw/2 (v_rms units, src/ProcessDSMC.cpp:1808-1818)
+    m = D.Mesh.box_hexes(1)
+    p = D.params_box(0.05)
+    p.wall_model = D.WALL_DIFFUSE
+    p.T_wall = 300.0
+    p.dr = 2.0
+    D.derive(m, p)
+    p.dtime = 0.02
+    n = 200000
+    parts = D.Particles(n)
+    parts.x[:] = 0.005
+    parts.y[:] = 0.5
+    parts.z[:] = 0.5
+    parts.vx[:] = -1.0
+    parts.id[:] = np.arange(n)
+    o = O.Oracle(m, p, seed=4)
+    o.set_particles(parts)
+    o.step(1, D.STAGE_MOVE | D.STAGE_SORT)
+    q = o.particles()
+    assert o.stats()["wall_hits"] == n
+    Tw = 300.0 / 273.15
+    tol = 5 / math.sqrt(n)
+    assert q.vx.min() > 0
+    assert abs(q.vx.mean() - math.sqrt(math.pi * Tw) / 2) < tol
+    assert abs(q.vy.mean()) < tol and abs(q.vz.mean()) < tol
+    assert abs(q.vy.var() - Tw / 2) < 4 * tol and abs(q.vz.var() - Tw / 2) < 4 * tol
+    assert abs(q.ir.mean() - Tw / 2) < 4 * tol
+
+
+def test_inflow_flux_matches_one_way_maxwellian():
+    # Bird (1994) eq. 4.22: inward number flux through a surface element with
+    # inward unit normal at angle theta to U: n/(2 sqrt(pi) beta) [exp(-s^2 cos^2)
+    # + sqrt(pi) s cos (1 + erf(s cos))], s = U beta.  Over S steps the emitted
+    # count per inlet cell equals S * rate up to the carried remainder (< 1).
+    m = D.Mesh.capsule(a=5, c=10, layers=4)
+    p = D.params_apollo(monatomic=True)
+    p.Npinitial = 5
+    D.derive(m, p)
+    a = m.arrays()
+    o = O.Oracle(m, p, seed=2)
+    o.set_particles(D.Particles(0))
+    S = 12
+    for s in range(1, S + 1):
+        o.step(s, D.STAGE_EMIT | D.STAGE_SORT)
+    got = o.stats()["emitted"]
+    beta = 1 / math.sqrt(2 * KB / p.mass * p.T_in)
+    U = np.array(p.u_inf) * p.v_rms
+    s_ = np.linalg.norm(U) * beta
+    inlet = np.flatnonzero(a["face_tag"] == D.TAG_INLET)
+    rate = 0.0
+    for f in inlet:
+        nodes = a["node_xyz"][a["face_nodes"][f][: a["face_type"][f]]]
+        # flux through each triangle of the quad with its own normal
+        tris = [(0, 1, 2), (0, 2, 3)] if a["face_split"][f] == 0 else [(0, 1, 3), (1, 2, 3)]
+        for t in tris:
+            P0, P1, P2 = nodes[list(t)]
+            nv = np.cross(P1 - P0, P2 - P0)
+            area = 0.5 * np.linalg.norm(nv)
+            nin = -nv / np.linalg.norm(nv) * np.sign(np.dot(nv, a["face_normal"][f]))
+            c = np.dot(U, nin) / np.linalg.norm(U)
+            sc = s_ * c
+            flux = p.n_ref / (2 * math.sqrt(math.pi) * beta) * (math.exp(-sc * sc) + math.sqrt(math.pi) * sc *
+                                                                 (1 + math.erf(sc)))
+            rate += flux * area * p.dtime / p.v_rms / p.Neff
+    want = rate * S
+    ninlet_cells = len(np.unique(a["face_cells"][inlet, 0]))
+    assert abs(got - want) <= ninlet_cells, (got, want)
+
+
+def test_sampling_matches_peculiar_velocity_definitions():
+    # writeLocalMacro via statistic_macroPre against the moment definitions in
+    # peculiar velocity c = v - u: T = 2/3 <c^2>, sigma_ij = 2 rho (<c_i c_j> -
+    # delta_ij <c^2>/3), q_i = rho <c_i c^2>, q_r,i = 2 rho <c_i e_r>, Tr = 4/dr <e_r>
+    m = D.Mesh.box_hexes(2)
+    p = D.params_box(0.05)
+    p.dr = 2.0
+    D.derive(m, p)
+    rng = np.random.default_rng(3)
+    n = 800
+    parts = D.Particles(n)
+    parts.cell[:] = rng.integers(0, m.ncell, n)
+    parts.x[:], parts.y[:], parts.z[:] = 0.5, 0.5, 0.5
+    parts.vx[:] = rng.normal(0.7, 1.0, n)
+    parts.vy[:] = rng.gamma(2.0, 0.5, n)
+    parts.vz[:] = rng.normal(-0.2, 0.6, n)
+    parts.ir[:] = rng.exponential(0.8, n)
+    parts.id[:] = np.arange(n)
+    o = O.Oracle(m, p, seed=1)
+    o.set_particles(parts)
+    o.statistic_pre()
+    loc = o.macro(D.MACRO_LOCAL)
+    vol = m.arrays()["cell_volume"]
+    for c in range(m.ncell):
+        sel = parts.cell == c
+        v = np.stack([parts.vx[sel], parts.vy[sel], parts.vz[sel]], 1)
+        er = parts.ir[sel]
+        N = sel.sum()
+        rho = p.Neff / vol[c] * N / p.n_ref
+        u = v.mean(0)
+        cc = v - u
+        c2 = (cc ** 2).sum(1)
+        T = 2 / 3 * c2.mean()
+        P = cc.T @ cc / N
+        sig = 2 * rho * (P - np.eye(3) * c2.mean() / 3)
+        q = rho * (cc * c2[:, None]).mean(0)
+        qr = 2 * rho * (cc * er[:, None]).mean(0)
+        want = [rho, *u, T, sig[0, 0], sig[0, 1], sig[1, 1], sig[1, 2], sig[2, 2], sig[2, 0], *q,
+                4 / p.dr * er.mean(), *qr]
+        np.testing.assert_allclose(loc[c], want, rtol=1e-10, atol=1e-12)
+
+
+def test_sampling_windows_closed_form():
+    # frozen particles (sample only): stepsum = s m up to NSCHEME, then the
+    # window recurrence keeps NSCHEME m; steady after NSS averages to the
+    # single-step fields; the interval record every NGSIS equals them too.
+    m = D.Mesh.box_hexes(2)
+    p = D.params_box(0.05)
+    p.dr = 2.0
+    p.NSCHEME, p.NSS, p.Nevery, p.Nrepeat = 5, 4, 1, 10
+    D.derive(m, p)
+    parts = D.fill_freestream(m, p, ppc=30.0, temperature=1.0, seed=8)
+    parts.ir[:] = np.random.default_rng(1).exponential(0.5, parts.n)
+    o = O.Oracle(m, p, seed=1)
+    o.set_particles(parts)
+    o.statistic_pre()
+    one = o.macro(D.MACRO_LOCAL)
+    for s in range(1, 4):
+        o.step(s, D.STAGE_SAMPLE)
+    loc = o.macro(D.MACRO_LOCAL)
+    # rho and every field carrying a factor rho (stress, heat fluxes) see the
+    # partly filled window: 3 of NSCHEME = 5 samples; u, T, Tr do not
+    scaled = [0, 5, 6, 7, 8, 9, 10, 11, 12, 13, 15, 16, 17]
+    want = one.copy()
+    want[:, scaled] *= 3 / 5
+    np.testing.assert_allclose(loc, want, rtol=1e-12, atol=1e-14)
+    for s in range(4, 31):
+        o.step(s, D.STAGE_SAMPLE)
+    np.testing.assert_allclose(o.macro(D.MACRO_LOCAL), one, rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(o.macro(D.MACRO_FINAL), one, rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(o.macro(D.MACRO_RECORD), one, rtol=1e-11, atol=1e-13)
+    # the interval accumulator restarts after every exchange step (s = 30)
+    assert np.all(o.accum(2) == 0.0)
+    np.testing.assert_allclose(o.accum(1)[:, 0], 26 * np.bincount(parts.cell, minlength=m.ncell))
+
+
+def test_sparse_cell_state_machine():
+    # a cell with fewer than 3 samples in an interval becomes "accumulating";
+    # its interval sums are no longer reset and its record averages over the
+    # steps accumulated since (src/ProcessDSMC.cpp:2798-2858).  NGSIS = 1 here.
+    m = D.Mesh.box_hexes(2)
+    p = D.params_box(0.05)
+    p.NSCHEME, p.NSS, p.Nevery, p.Nrepeat = 5, 100, 1, 1
+    D.derive(m, p)
+    parts = D.Particles(6)
+    parts.cell[:] = [0, 0, 1, 1, 1, 1]
+    parts.x[:] = 0.25
+    parts.vx[:] = [0.3, -0.1, 1.0, -1.0, 0.5, 0.1]
+    parts.id[:] = np.arange(6)
+    o = O.Oracle(m, p, seed=1)
+    o.set_particles(parts)
+    unit = p.Neff / m.arrays()["cell_volume"] / p.n_ref
+    o.step(1, D.STAGE_SAMPLE)
+    sc = o.cell_scalars()
+    # cell 0: 2 samples < 3 -> accumulating; cell 1: 4 samples; empty cells: 0 < 3
+    assert list(sc["sparse_state"]) == [1, 0, 1, 1, 1, 1, 1, 1]
+    rec = o.macro(D.MACRO_RECORD)
+    # 2 samples over max(1, Nrepeat) = 1 step; u = 0.1; T = 2/3 <c^2> = 2/3 * 0.04
+    assert rec[0, 0] == pytest.approx(2 * unit[0]) and rec[0, 1] == pytest.approx(0.1)
+    assert rec[0, 4] == pytest.approx(2 / 3 * 0.04)
+    assert rec[1, 0] == pytest.approx(4 * unit[1]) and rec[1, 1] == pytest.approx(0.6 / 4)
+    assert np.all(rec[2:] == 0.0)  # empty cells are invalid -> zeros
+    acc = o.accum(2)
+    assert acc[0, 0] == 2 and acc[1, 0] == 0  # sparse sums kept, normal reset
+    o.step(2, D.STAGE_SAMPLE)
+    assert o.cell_scalars()["sparse_steps"][0] == 2 and o.accum(2)[0, 0] == 4
+    rec = o.macro(D.MACRO_RECORD)
+    assert rec[0, 0] == pytest.approx(4 / 2 * unit[0]) and rec[0, 1] == pytest.approx(0.1)
+    # out2dat shows record for sparse cells and before NSS
+    out = o.macro(D.MACRO_OUTPUT)
+    assert np.array_equal(out, rec)
+
+
+# ---- the C ABI ------------------------------------------------------------------------------
+def test_library_exports_every_declared_symbol():
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "dsmc_engine.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = sorted(set(re.findall(r"\b(dsmc_[a-z0-9_]+)\s*\(", hdr)))
+    assert len(names) >= 30
+    L = ctypes.CDLL(D.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(D.EXPORTED)
+
+
+def test_params_and_mesh_errors_are_reported():
+    with pytest.raises(D.DsmcError):
+        D.Mesh.box_tets(0)
+    m = D.Mesh.box_tets(2)
+    p = D.params_box(0.05)
+    p.Kn = -1.0
+    with pytest.raises(D.DsmcError):
+        D.derive(m, p)
