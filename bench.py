@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB plain copy kernel (counter calibration, achievable HBM GB/s)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU per case; multi-GPU runs are not part of this build "
@@ -135,6 +137,10 @@ def main():
         sys.exit(2)
 
     D.lib()  # fail loudly if the engine library is missing
+    copy_gbs = None
+    if args.copy_probe:
+        copy_gbs = D.copy_probe(1 << 30, 5)
+        log(f"[bench] copy probe: 1 GiB x 5, {copy_gbs:.0f} GB/s (read + write)")
     t0 = time.perf_counter()
     mesh, p = make_case(args.n, args.ppc, False)
     parts = D.fill_freestream(mesh, p, ppc=args.ppc, temperature=1.0, seed=20261203)
@@ -199,6 +205,7 @@ def main():
                                              "speedup_vs_fp64": round(mixed["value"] / fp64["value"], 3)},
         "gpu_vs_cpu": None if cpu is None else round(fp64["value"] / cpu["value"], 1),
         "footprint_GB": round(fp64["footprint_GB"], 2),
+        "copy_probe_GBs": None if copy_gbs is None else round(copy_gbs, 1),
     }
     print(json.dumps(out), flush=True)
 

@@ -73,12 +73,25 @@ struct CrossLog {
   int32_t* count;
 };
 
+// Particle records: position + cell + id (32 B) and velocity + Ir (32 B FP64,
+// 16 B mixed).  A record is read or written with two (one) 16-byte loads, so
+// the regroup moves each particle as two contiguous pieces instead of nine
+// scattered scalars, and collide/sample reads only the velocity records.
+struct alignas(32) PosRec {
+  double x, y, z;
+  int32_t cell;
+  uint32_t id;
+};
+
+template <class VT>
+struct alignas(4 * sizeof(VT)) VelRec {
+  VT vx, vy, vz, ir;
+};
+
 template <class VT>
 struct PBuf {
-  double *x, *y, *z;
-  VT *vx, *vy, *vz, *ir;
-  int32_t* cell;
-  uint32_t* id;
+  PosRec* pos;
+  VelRec<VT>* vel;
 };
 
 // Accumulator families (stepsum, steady, stepinter), structure of arrays

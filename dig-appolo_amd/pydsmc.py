@@ -85,7 +85,7 @@ EXPORTED = [
     "dsmc_particle_count", "dsmc_step", "dsmc_run", "dsmc_track", "dsmc_track_log_enable", "dsmc_track_log_get",
     "dsmc_macro_get", "dsmc_statistic_pre", "dsmc_accum_get", "dsmc_cell_scalars_get", "dsmc_surface_get",
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
-    "dsmc_engine_footprint", "dsmc_device_math_check", "dsmc_last_error",
+    "dsmc_engine_footprint", "dsmc_device_math_check", "dsmc_copy_probe", "dsmc_last_error",
 ]
 
 _lib = None
@@ -128,6 +128,13 @@ def device_math(u: np.ndarray) -> np.ndarray:
     check(lib().dsmc_device_math_check(_ptr(u, C.c_double), C.c_int64(len(u)), _ptr(out, C.c_double)),
           "device_math_check")
     return out
+
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """Achieved HBM GB/s (read + write) of a plain device copy of nbytes."""
+    g = C.c_double()
+    check(lib().dsmc_copy_probe(C.c_int64(nbytes), reps, C.byref(g)), "copy_probe")
+    return g.value
 
 
 class Mesh:

@@ -317,6 +317,11 @@ int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
  * check that device and host arithmetic agree bit for bit.  No counterpart
  * in the reference. */
 int dsmc_device_math_check(const double *u, int64_t n, double *out);
+/* Diagnostic: device-to-device copy of `bytes` with a plain streaming kernel
+ * (k_copy_probe), `reps` times; returns the achieved GB/s (read + write).
+ * Used to calibrate HBM traffic counters and the achievable bandwidth.  No
+ * counterpart in the reference. */
+int dsmc_copy_probe(int64_t bytes, int32_t reps, double *gbs);
 
 const char *dsmc_last_error(void);
 
