@@ -58,6 +58,18 @@ def make_case(n: int, ppc: float, mixed: bool):
     return mesh, p
 
 
+def make_apollo(ppc: float, mixed: bool):
+    """Apollo capsule, plain DSMC: surrogate 372,500-cell hex O-grid, monatomic
+    VHS, Ma 5, Kn 0.01, AoA 30, T_in 142.2 K, diffuse wall at 300 K, inflow."""
+    mesh = D.Mesh.capsule(25, 62, 50, 3.912)
+    p = D.params_apollo(monatomic=True)
+    p.Npinitial = int(round(ppc))
+    p.NSS = 250
+    p.precision = D.PREC_MIXED if mixed else D.PREC_FP64
+    D.derive(mesh, p)
+    return mesh, p
+
+
 def run_engine(mesh, p, parts, steps, warmup, seed=20261203):
     eng = D.Engine(mesh, p, seed=seed)
     eng.upload(parts)
@@ -124,6 +136,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64, help="box resolution (hexes per side)")
+    ap.add_argument("--workload", choices=["box", "apollo"], default="box")
     ap.add_argument("--ppc", type=float, default=50.0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -142,10 +155,17 @@ def main():
         copy_gbs = D.copy_probe(1 << 30, 5)
         log(f"[bench] copy probe: 1 GiB x 5, {copy_gbs:.0f} GB/s (read + write)")
     t0 = time.perf_counter()
-    mesh, p = make_case(args.n, args.ppc, False)
-    parts = D.fill_freestream(mesh, p, ppc=args.ppc, temperature=1.0, seed=20261203)
-    log(f"[bench] box {args.n}^3x6 tets: {mesh.ncell} cells, {parts.n} particles "
-        f"(set-up {time.perf_counter() - t0:.1f} s)")
+    if args.workload == "apollo":
+        if args.ppc == 50.0:
+            args.ppc = 20.0
+        mesh, p = make_apollo(args.ppc, False)
+        parts = D.fill_freestream(mesh, p, ppc=args.ppc, seed=20261203)
+        ntri, label = 12.0, "apollo"
+    else:
+        mesh, p = make_case(args.n, args.ppc, False)
+        parts = D.fill_freestream(mesh, p, ppc=args.ppc, temperature=1.0, seed=20261203)
+        ntri, label = 4.0, f"box {args.n}^3x6 tets"
+    log(f"[bench] {label}: {mesh.ncell} cells, {parts.n} particles (set-up {time.perf_counter() - t0:.1f} s)")
     fp64 = run_engine(mesh, p, parts, args.steps, args.warmup)
     log(f"[bench] fp64: {fp64['value']:.4g} particle-steps/s, {fp64['ms_per_step']:.2f} ms/step")
 
@@ -163,7 +183,6 @@ def main():
                                                                    "sort_gather")),
               "collide_sample": kt["collide_sample"][0]}
     dom = max(groups, key=groups.get)
-    ntri = 4.0  # triangles per tet
     per_particle = alg_bytes(dom, 64, 32, 80.0 * ntri, args.ppc)
     nmean = fp64["particle_steps"] / args.steps
     launch_ms = groups[dom] / args.steps
@@ -192,9 +211,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Maxwellian fill of a generated mesh)",
-        "config": {"workload": f"equilibrium box {args.n}^3 hexes x 6 tets, hard spheres, Kn 0.05, "
-                               f"{args.ppc:g} particles/cell, specular walls, full step "
-                               "(emit+move+sort+collide+sample)",
+        "config": {"workload": (f"equilibrium box {args.n}^3 hexes x 6 tets, hard spheres, Kn 0.05, "
+                                f"{args.ppc:g} particles/cell, specular walls, full step "
+                                "(emit+move+sort+collide+sample)") if args.workload == "box" else
+                               ("Apollo capsule surrogate hex O-grid (372,500 cells), monatomic VHS, Ma 5, "
+                                f"Kn 0.01, AoA 30, diffuse wall 300 K, inflow, Npinitial {args.ppc:g}, full step"),
                    "cells": mesh.ncell, "particles": parts.n, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -52,8 +52,8 @@ struct Counters {
   int32_t overflow;  // capacity exceeded (sticky)
   uint32_t next_id;
   int32_t last_step;
-  int32_t log_count;
-  int32_t id_base;   // first id of this step's new inflow particles
+  int32_t park_count;  // particles parked at a wall by the move kernel this step
+  int32_t id_base;     // first id of this step's new inflow particles
   unsigned long long emitted, removed, dropped, wall_hits, crossings, coll_pairs, collisions, particle_steps;
 };
 
@@ -86,6 +86,12 @@ struct alignas(32) PosRec {
 template <class VT>
 struct alignas(4 * sizeof(VT)) VelRec {
   VT vx, vy, vz, ir;
+};
+
+// A walk stopped at a wall by the move kernel, finished by the wall kernel.
+struct alignas(16) Park {
+  double dtleft;
+  int32_t i, cell, crossfid, tri, seq, iter, hits, ncross;
 };
 
 template <class VT>
