@@ -99,13 +99,19 @@ def run_engine(mesh, p, parts, steps, warmup, seed=20261203):
     return res
 
 
-def cpu_baseline(threads: int, seconds_target: float = 15.0):
+def cpu_baseline(threads: int, workload: str = "box", ppc: float = 20.0, seconds_target: float = 15.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
 
-    n = 24
-    mesh, p = make_case(n, 50.0, False)
-    parts = D.fill_freestream(mesh, p, ppc=50.0, temperature=1.0, seed=7)
+    if workload == "apollo":
+        mesh, p = make_apollo(ppc, False)
+        parts = D.fill_freestream(mesh, p, ppc=ppc, seed=7)
+        what = f"C++ oracle, Apollo surrogate ({mesh.ncell} cells), Npinitial {ppc:g} ({parts.n} particles)"
+    else:
+        n = 24
+        mesh, p = make_case(n, 50.0, False)
+        parts = D.fill_freestream(mesh, p, ppc=50.0, temperature=1.0, seed=7)
+        what = f"C++ oracle, {n}^3x6-tet box ({mesh.ncell} cells), 50 ppc ({parts.n} particles)"
     O.set_threads(threads)
     orc = O.Oracle(mesh, p, seed=7)
     orc.set_particles(parts)
@@ -125,8 +131,7 @@ def cpu_baseline(threads: int, seconds_target: float = 15.0):
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"C++ oracle, {n}^3x6-tet box ({mesh.ncell} cells), 50 ppc ({parts.n} particles), "
-                  f"{steps} full steps in {dt:.1f} s, {threads} threads",
+        "sample": f"{what}, {steps} full steps in {dt:.1f} s, {threads} threads",
     }
 
 
@@ -183,19 +188,19 @@ def main():
                                                                    "sort_gather")),
               "collide_sample": kt["collide_sample"][0]}
     dom = max(groups, key=groups.get)
-    per_particle = alg_bytes(dom, 64, 32, 80.0 * ntri, args.ppc)
+    per_particle = alg_bytes(dom, 64, 32, 80.0 * ntri, nmean_ppc := fp64["particle_steps"] / args.steps / mesh.ncell)
     nmean = fp64["particle_steps"] / args.steps
     launch_ms = groups[dom] / args.steps
     achieved = per_particle * nmean / (launch_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                 "bytes_per_particle": per_particle, "launch_ms": round(launch_ms, 4)}
-    step_bytes = sum(alg_bytes(k, 64, 32, 80.0 * ntri, args.ppc) for k in ("move", "sort", "collide_sample"))
+    step_bytes = sum(alg_bytes(k, 64, 32, 80.0 * ntri, nmean_ppc) for k in ("move", "sort", "collide_sample"))
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(threads)
+        cpu = cpu_baseline(threads, args.workload, args.ppc)
         log(f"[bench] cpu baseline: {cpu['value']:.4g} particle-steps/s on {threads} threads")
 
     out = {
