@@ -54,6 +54,8 @@ struct Counters {
   int32_t last_step;
   int32_t park_count;  // particles parked at a wall by the move kernel this step
   int32_t id_base;     // first id of this step's new inflow particles
+  int32_t wide_n;      // cells queued for the one-wave-per-cell collide+sample kernel
+  int32_t wide_ticket; // next queued cell to hand out in that kernel
   unsigned long long emitted, removed, dropped, wall_hits, crossings, coll_pairs, collisions, particle_steps;
 };
 
@@ -65,6 +67,7 @@ struct KParams {
   double dr, n_ref;
   int32_t wall_model, rot, surface, NSS, NSCHEME, Nevery, Nrepeat, ncell;
   int32_t capacity, npatch, ninlet_cells, max_iter;
+  int32_t wide_min;  // cells with more particles go one wave per cell (< 0: never)
 };
 
 struct CrossLog {

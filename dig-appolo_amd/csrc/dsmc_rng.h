@@ -89,7 +89,14 @@ struct Stream {
       ++block;
       used = 0u;
     }
-    return buf[used++];
+    // pop the front word; constant indices only, so the buffer stays in
+    // registers on the GPU
+    const uint32_t x = buf[0];
+    buf[0] = buf[1];
+    buf[1] = buf[2];
+    buf[2] = buf[3];
+    ++used;
+    return x;
   }
   // Uniform double in (0, 1]: 53 random bits, offset by one unit so that
   // log() of a draw is always finite.  Exact integer arithmetic, so the

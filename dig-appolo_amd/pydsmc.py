@@ -71,6 +71,13 @@ class KernelTimes(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("calls", C.c_int64 * 8)]
 
 
+TUNE_KEEP = -2147483648
+
+
+class Tuning(C.Structure):
+    _fields_ = [("wide_min", C.c_int32), ("wide_grid", C.c_int32), ("reserved", C.c_int32 * 6)]
+
+
 class CrossEvent(C.Structure):
     _fields_ = [("id", C.c_uint32), ("step", C.c_int32), ("seq", C.c_int32), ("face", C.c_int32),
                 ("tri", C.c_int32), ("kind", C.c_int32)]
@@ -85,7 +92,8 @@ EXPORTED = [
     "dsmc_particle_count", "dsmc_step", "dsmc_run", "dsmc_track", "dsmc_track_log_enable", "dsmc_track_log_get",
     "dsmc_macro_get", "dsmc_statistic_pre", "dsmc_accum_get", "dsmc_cell_scalars_get", "dsmc_surface_get",
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
-    "dsmc_engine_footprint", "dsmc_device_math_check", "dsmc_copy_probe", "dsmc_last_error",
+    "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_device_math_check",
+    "dsmc_copy_probe", "dsmc_last_error",
 ]
 
 _lib = None
@@ -384,3 +392,15 @@ class Engine:
         b = C.c_int64()
         check(lib().dsmc_engine_footprint(self.h, C.byref(b)), "footprint")
         return b.value
+
+    def set_tuning(self, wide_min: int | None = None, wide_grid: int | None = None):
+        """Launch tuning; results are bit-identical for every setting."""
+        t = Tuning()
+        t.wide_min = TUNE_KEEP if wide_min is None else int(wide_min)
+        t.wide_grid = TUNE_KEEP if wide_grid is None else int(wide_grid)
+        check(lib().dsmc_engine_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(lib().dsmc_engine_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid}

@@ -311,6 +311,23 @@ int dsmc_last_run_ms(dsmc_engine *e, double *ms);
 /* Bytes of device memory the engine holds. */
 int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
 
+/* Launch tuning (no reference counterpart: the reference runs one CPU thread
+ * per rank).  None of these change results, which are bit-identical for every
+ * setting; they only move work between kernels.
+ *   wide_min   cells with more particles than this are collided and sampled
+ *              by one wavefront per cell instead of one thread; < 0: never
+ *              (default 64)
+ *   wide_grid  workgroups of that one-wave-per-cell kernel (default 1024)
+ * Fields set to DSMC_TUNE_KEEP are left unchanged. */
+#define DSMC_TUNE_KEEP (-2147483647 - 1)
+typedef struct dsmc_tuning {
+  int32_t wide_min;
+  int32_t wide_grid;
+  int32_t reserved[6];
+} dsmc_tuning;
+int dsmc_engine_set_tuning(dsmc_engine *e, const dsmc_tuning *t);
+int dsmc_engine_get_tuning(dsmc_engine *e, dsmc_tuning *t);
+
 /* Diagnostic: evaluates on the device, for each input u in (0, 1], the
  * shared deterministic functions the kernels use — out[6i..6i+5] =
  * log(u), exp(-20u), sin(2 pi u), cos(2 pi u), sqrt(u), u^0.26.  Lets a test

@@ -59,6 +59,13 @@ def _case(kind, seed=3):
         D.derive(mesh, p)
         p.NSS = 2
         parts = D.fill_freestream(mesh, p, ppc=6.0, seed=seed)
+    elif kind == "box_dense":
+        # > 64 particles per cell: strided moment sums, many candidates per cell
+        mesh = D.Mesh.box_tets(3)
+        p = D.params_box(0.05)
+        p.Npinitial = 100
+        D.derive(mesh, p)
+        parts = D.fill_freestream(mesh, p, ppc=100.0, temperature=1.0, seed=seed)
     elif kind == "box_hex":
         mesh = D.Mesh.box_hexes(6)
         p = D.params_box(0.05)
@@ -97,11 +104,17 @@ def test_track_parity(kind):
         assert gs[k] == cs[k], (k, gs[k], cs[k])
 
 
-@pytest.mark.parametrize("kind", ["box_tets", "capsule", "capsule_tets"])
-def test_full_step_parity(kind):
+# wide_min: None = default routing; 0 = every cell one wave per cell; -1 =
+# every cell one thread per cell
+@pytest.mark.parametrize("kind,wide_min", [("box_tets", None), ("capsule", None), ("capsule_tets", None),
+                                           ("box_dense", None), ("box_dense", -1), ("capsule", 0),
+                                           ("box_dense", 0), ("box_dense", 20)])
+def test_full_step_parity(kind, wide_min):
     mesh, p, parts = _case(kind)
-    print(f"[full_step_parity] {kind} cells={mesh.ncell} particles={parts.n}", flush=True)
+    print(f"[full_step_parity] {kind} wide_min={wide_min} cells={mesh.ncell} particles={parts.n}", flush=True)
     eng = D.Engine(mesh, p, seed=21)
+    if wide_min is not None:
+        eng.set_tuning(wide_min=wide_min)
     eng.upload(parts)
     orc = O.Oracle(mesh, p, seed=21)
     orc.set_particles(parts)
