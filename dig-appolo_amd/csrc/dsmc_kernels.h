@@ -22,12 +22,33 @@ namespace dsmc {
 // One triangle of a cell's face cache (FaceTriCache, src/ProcessDSMC.h:109-116):
 // vertex A, edges e1 = B - A, e2 = C - A, the face it belongs to and its tag
 // (-1 for triangular faces, 1/2 for the halves of a split quad).  80 bytes.
+// One triangle of a cell's face (FaceTriCache entry): vertex A, edges
+// e1 = B - A, e2 = C - A, and two packed words so that the walk decides what
+// a crossing does without reading the face table:
+//   face_tag = face | tagcode << 29   (tagcode 0: triangle face, tag -1;
+//                                      1, 2: half 1 or 2 of a split quad)
+//   link     = the cell across the face, if the face is an interface with a
+//              cell on the other side; otherwise -1 - role (role != interface)
+//              or LINK_OPEN_INTERFACE (interface tag on a boundary face)
 struct TriRec {
   double ax, ay, az, e1x, e1y, e1z, e2x, e2y, e2z;
-  int32_t face, tag;
+  int32_t face_tag, link;
 };
 
 enum FaceRole : int32_t { ROLE_NONE = 0, ROLE_INTERFACE = 1, ROLE_INLET = 2, ROLE_OUTLET = 3, ROLE_WALL = 4 };
+
+constexpr int32_t TRI_FACE_BITS = 29;
+constexpr int32_t TRI_FACE_MASK = (1 << TRI_FACE_BITS) - 1;
+constexpr int32_t LINK_OPEN_INTERFACE = -1 - ROLE_INTERFACE;
+
+inline __host__ __device__ int32_t tri_pack(int32_t face, int32_t tag) {
+  return face | ((tag < 0 ? 0 : tag) << TRI_FACE_BITS);
+}
+inline __host__ __device__ int32_t tri_face(const TriRec& t) { return t.face_tag & TRI_FACE_MASK; }
+inline __host__ __device__ int32_t tri_tag(const TriRec& t) {
+  const int32_t c = (int32_t)((uint32_t)t.face_tag >> TRI_FACE_BITS);
+  return c == 0 ? -1 : c;
+}
 
 // c0/c1 = the two cells (c1 = -1 on boundaries), role, wall slot (-1 if not a wall).
 struct FaceRec {
@@ -67,7 +88,8 @@ struct KParams {
   double dr, n_ref;
   int32_t wall_model, rot, surface, NSS, NSCHEME, Nevery, Nrepeat, ncell;
   int32_t capacity, npatch, ninlet_cells, max_iter;
-  int32_t wide_min;  // cells with more particles go one wave per cell (< 0: never)
+  int32_t wide_min;    // cells with more particles go one wave per cell (< 0: never)
+  int32_t move_chunk;  // particles per move workgroup
 };
 
 struct CrossLog {

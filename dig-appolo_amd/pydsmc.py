@@ -75,7 +75,8 @@ TUNE_KEEP = -2147483648
 
 
 class Tuning(C.Structure):
-    _fields_ = [("wide_min", C.c_int32), ("wide_grid", C.c_int32), ("reserved", C.c_int32 * 6)]
+    _fields_ = [("wide_min", C.c_int32), ("wide_grid", C.c_int32), ("move_chunk", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class CrossEvent(C.Structure):
@@ -393,14 +394,15 @@ class Engine:
         check(lib().dsmc_engine_footprint(self.h, C.byref(b)), "footprint")
         return b.value
 
-    def set_tuning(self, wide_min: int | None = None, wide_grid: int | None = None):
+    def set_tuning(self, wide_min: int | None = None, wide_grid: int | None = None,
+                   move_chunk: int | None = None):
         """Launch tuning; results are bit-identical for every setting."""
         t = Tuning()
-        t.wide_min = TUNE_KEEP if wide_min is None else int(wide_min)
-        t.wide_grid = TUNE_KEEP if wide_grid is None else int(wide_grid)
+        for f, v in (("wide_min", wide_min), ("wide_grid", wide_grid), ("move_chunk", move_chunk)):
+            setattr(t, f, TUNE_KEEP if v is None else int(v))
         check(lib().dsmc_engine_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def tuning(self) -> dict:
         t = Tuning()
         check(lib().dsmc_engine_get_tuning(self.h, C.byref(t)), "get_tuning")
-        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid}
+        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk}

@@ -318,12 +318,16 @@ int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
  *              by one wavefront per cell instead of one thread; < 0: never
  *              (default 64)
  *   wide_grid  workgroups of that one-wave-per-cell kernel (default 1024)
+ *   move_chunk particles per workgroup of the move kernel, whose lanes take
+ *              the chunk's particles one after another (64..1048576,
+ *              default 1024)
  * Fields set to DSMC_TUNE_KEEP are left unchanged. */
 #define DSMC_TUNE_KEEP (-2147483647 - 1)
 typedef struct dsmc_tuning {
   int32_t wide_min;
   int32_t wide_grid;
-  int32_t reserved[6];
+  int32_t move_chunk;
+  int32_t reserved[5];
 } dsmc_tuning;
 int dsmc_engine_set_tuning(dsmc_engine *e, const dsmc_tuning *t);
 int dsmc_engine_get_tuning(dsmc_engine *e, dsmc_tuning *t);

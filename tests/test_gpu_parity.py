@@ -82,11 +82,16 @@ def _case(kind, seed=3):
     return mesh, p, parts
 
 
-@pytest.mark.parametrize("kind", ["box_tets", "box_hex", "capsule", "capsule_tets"])
-def test_track_parity(kind):
+# move_chunk: particles per move workgroup (lanes refill from the chunk); the
+# schedule must not change any walk
+@pytest.mark.parametrize("kind,move_chunk", [("box_tets", None), ("box_hex", None), ("capsule", None),
+                                             ("capsule_tets", None), ("box_tets", 64), ("capsule", 256),
+                                             ("capsule_tets", 4096), ("box_hex", 100)])
+def test_track_parity(kind, move_chunk):
     mesh, p, parts = _case(kind)
-    print(f"[track_parity] {kind} cells={mesh.ncell} particles={parts.n}", flush=True)
+    print(f"[track_parity] {kind} move_chunk={move_chunk} cells={mesh.ncell} particles={parts.n}", flush=True)
     eng = D.Engine(mesh, p, seed=11)
+    eng.set_tuning(move_chunk=move_chunk)
     eng.upload(parts)
     orc = O.Oracle(mesh, p, seed=11)
     orc.set_particles(parts)
@@ -165,3 +170,20 @@ def test_stagewise_matches_full_step():
     for k in a:
         assert a[k] == b[k], (k, a[k], b[k])
     assert np.array_equal(split.macro(D.MACRO_OUTPUT), full.macro(D.MACRO_OUTPUT), equal_nan=True)
+
+
+def test_stage_order_guard():
+    """A move needs the particles grouped by cell: a second move without the
+    regroup in between, or a collide after a move, is refused with an error
+    instead of walking stale cells."""
+    mesh, p, parts = _case("box_tets")
+    eng = D.Engine(mesh, p, seed=3)
+    eng.upload(parts)
+    eng.step(1, D.STAGE_MOVE)
+    with pytest.raises(D.DsmcError, match="regroup"):
+        eng.step(2, D.STAGE_MOVE)
+    with pytest.raises(D.DsmcError, match="regroup"):
+        eng.step(1, D.STAGE_COLLIDE)
+    eng.step(1, D.STAGE_SORT)
+    eng.step(2, D.STAGE_ALL)
+    assert eng.stats()["particles"] > 0
