@@ -320,7 +320,7 @@ int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
  *   wide_grid  workgroups of that one-wave-per-cell kernel (default 1024)
  *   move_chunk particles per workgroup of the move kernel, whose lanes take
  *              the chunk's particles one after another (64..1048576,
- *              default 1024)
+ *              default 2048)
  * Fields set to DSMC_TUNE_KEEP are left unchanged. */
 #define DSMC_TUNE_KEEP (-2147483647 - 1)
 typedef struct dsmc_tuning {
