@@ -70,10 +70,16 @@ def make_apollo(ppc: float, mixed: bool):
     return mesh, p
 
 
-def run_engine(mesh, p, parts, steps, warmup, seed=20261203):
+def run_engine(mesh, p, parts, steps, warmup, seed=20261203, autotune=True):
     eng = D.Engine(mesh, p, seed=seed)
     eng.upload(parts)
-    del_ms = 0.0
+    tuning, tune_steps = eng.tuning(), 0
+    if autotune:
+        # set-up: the autotuner times live steps (results do not depend on the
+        # tuning it picks), then the case restarts from the same particles
+        tuning, tune_steps = eng.autotune(1, 2)
+        eng.upload(parts)
+        log(f"[bench] autotune ({tune_steps} steps): {tuning}")
     if warmup > 0:
         eng.run(1, warmup)
     st0 = eng.stats()
@@ -93,9 +99,10 @@ def run_engine(mesh, p, parts, steps, warmup, seed=20261203):
         "kernels": kt,
         "stats": st1,
         "footprint_GB": eng.footprint() / 1e9,
+        "tuning": tuning,
+        "tune_steps": tune_steps,
     }
     eng.close()
-    del del_ms
     return res
 
 
@@ -145,6 +152,7 @@ def main():
     ap.add_argument("--ppc", type=float, default=50.0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB plain copy kernel (counter calibration, achievable HBM GB/s)")
@@ -171,14 +179,14 @@ def main():
         parts = D.fill_freestream(mesh, p, ppc=args.ppc, temperature=1.0, seed=20261203)
         ntri, label = 4.0, f"box {args.n}^3x6 tets"
     log(f"[bench] {label}: {mesh.ncell} cells, {parts.n} particles (set-up {time.perf_counter() - t0:.1f} s)")
-    fp64 = run_engine(mesh, p, parts, args.steps, args.warmup)
+    fp64 = run_engine(mesh, p, parts, args.steps, args.warmup, autotune=not args.no_autotune)
     log(f"[bench] fp64: {fp64['value']:.4g} particle-steps/s, {fp64['ms_per_step']:.2f} ms/step")
 
     mixed = None
     if not args.no_mixed:
         pm = p.copy()
         pm.precision = D.PREC_MIXED
-        mixed = run_engine(mesh, pm, parts, args.steps, args.warmup)
+        mixed = run_engine(mesh, pm, parts, args.steps, args.warmup, autotune=not args.no_autotune)
         log(f"[bench] mixed: {mixed['value']:.4g} particle-steps/s, {mixed['ms_per_step']:.2f} ms/step")
 
     # roofline of the dominant kernel group, from the HIP events around each
@@ -231,6 +239,7 @@ def main():
                                              "speedup_vs_fp64": round(mixed["value"] / fp64["value"], 3)},
         "gpu_vs_cpu": None if cpu is None else round(fp64["value"] / cpu["value"], 1),
         "footprint_GB": round(fp64["footprint_GB"], 2),
+        "tuning": {"fp64": fp64["tuning"], "mixed": None if mixed is None else mixed["tuning"]},
         "copy_probe_GBs": None if copy_gbs is None else round(copy_gbs, 1),
     }
     print(json.dumps(out), flush=True)

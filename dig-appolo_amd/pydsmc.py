@@ -93,7 +93,8 @@ EXPORTED = [
     "dsmc_particle_count", "dsmc_step", "dsmc_run", "dsmc_track", "dsmc_track_log_enable", "dsmc_track_log_get",
     "dsmc_macro_get", "dsmc_statistic_pre", "dsmc_accum_get", "dsmc_cell_scalars_get", "dsmc_surface_get",
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
-    "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_device_math_check",
+    "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
+    "dsmc_device_math_check",
     "dsmc_copy_probe", "dsmc_last_error",
 ]
 
@@ -401,6 +402,15 @@ class Engine:
         for f, v in (("wide_min", wide_min), ("wide_grid", wide_grid), ("move_chunk", move_chunk)):
             setattr(t, f, TUNE_KEEP if v is None else int(v))
         check(lib().dsmc_engine_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def autotune(self, first_step: int, steps_per_trial: int = 2) -> tuple[dict, int]:
+        """Tunes the launch parameters on live steps first_step, ...; returns
+        (tuning, steps used).  Results do not depend on the tuning."""
+        t = Tuning()
+        used = C.c_int32()
+        check(lib().dsmc_autotune(self.h, int(first_step), int(steps_per_trial), C.byref(t), C.byref(used)),
+              "autotune")
+        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk}, used.value
 
     def tuning(self) -> dict:
         t = Tuning()

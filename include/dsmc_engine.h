@@ -331,6 +331,15 @@ typedef struct dsmc_tuning {
 } dsmc_tuning;
 int dsmc_engine_set_tuning(dsmc_engine *e, const dsmc_tuning *t);
 int dsmc_engine_get_tuning(dsmc_engine *e, dsmc_tuning *t);
+/* Autotuner (no reference counterpart): picks move_chunk, wide_min and
+ * wide_grid for this case by timing the kernels on live steps first_step,
+ * first_step + 1, ... (the tuning never changes results, so these are
+ * ordinary steps of the run), sets the best and returns it.  *steps_used
+ * gets the number of steps run; continue the run from first_step +
+ * *steps_used.  steps_per_trial: 1..100 (2 is enough for cases of a few
+ * million particles or more). */
+int dsmc_autotune(dsmc_engine *e, int32_t first_step, int32_t steps_per_trial, dsmc_tuning *best,
+                  int32_t *steps_used);
 
 /* Diagnostic: evaluates on the device, for each input u in (0, 1], the
  * shared deterministic functions the kernels use — out[6i..6i+5] =

@@ -187,3 +187,25 @@ def test_stage_order_guard():
     eng.step(1, D.STAGE_SORT)
     eng.step(2, D.STAGE_ALL)
     assert eng.stats()["particles"] > 0
+
+
+def test_autotune_steps_are_ordinary_steps():
+    """The autotuner times live steps under different launch settings; the
+    run must end bit-identical to an untuned run of the same steps."""
+    mesh, p, parts = _case("capsule")
+    print(f"[autotune] capsule cells={mesh.ncell} particles={parts.n}", flush=True)
+    tuned = D.Engine(mesh, p, seed=9)
+    tuned.upload(parts)
+    tuning, used = tuned.autotune(1, 1)
+    assert used >= 10 and tuning["move_chunk"] in (512, 1024, 2048, 4096, 8192)
+    last = used + 2
+    tuned.run(used + 1, last)
+    plain = D.Engine(mesh, p, seed=9)
+    plain.upload(parts)
+    plain.run(1, last)
+    _same_particles(tuned.download(), plain.download())
+    for fam in (0, 1, 2):
+        assert np.array_equal(tuned.accum(fam), plain.accum(fam)), fam
+    a, b = tuned.stats(), plain.stats()
+    for k in a:
+        assert a[k] == b[k], (k, a[k], b[k])
