@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -133,6 +134,16 @@ def cpu_baseline(threads: int, workload: str = "box", ppc: float = 20.0, seconds
             break
     dt = time.perf_counter() - t0
     ps = orc.stats()["particle_steps"] - st0
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    what += f" on {model}"
     return {
         "value": ps / dt,
         "unit": "particle-steps/s",
@@ -140,6 +151,119 @@ def cpu_baseline(threads: int, workload: str = "box", ppc: float = 20.0, seconds
         "kind": "port",
         "sample": f"{what}, {steps} full steps in {dt:.1f} s, {threads} threads",
     }
+
+
+def _energy(q):
+    return 0.5 * float(np.sum(q.vx ** 2 + q.vy ** 2 + q.vz ** 2, dtype=np.float64)) + float(np.sum(q.ir))
+
+
+def validate_box(steps: int, n: int = 64, ppc: float = 50.0):
+    """Equilibrium box at full size: total energy drift and the NTC collision
+    count against the hard-sphere rate 1/2 N (N-1)/V Neff sigma <c_r> dt per
+    cell and step, <c_r> = 4 sqrt(kT/(pi m)) (Bird 1994, eqs. 1.10, 4.64),
+    summed over the cell counts of every step."""
+    kb = 1.3806e-23
+    out = {}
+    for prec in (D.PREC_FP64, D.PREC_MIXED):
+        mesh, p = make_case(n, ppc, prec == D.PREC_MIXED)
+        parts = D.fill_freestream(mesh, p, ppc=ppc, temperature=1.0, seed=20261203)
+        eng = D.Engine(mesh, p, seed=20261203)
+        eng.upload(parts)
+        eng.set_tuning(wide_min=-1, move_chunk=8192)
+        e0 = _energy(eng.download())
+        vol = mesh.arrays()["cell_volume"]
+        sigma = math.pi * p.d_ref ** 2
+        cr = 4.0 * math.sqrt(kb * p.T_ref / (math.pi * p.mass))
+        dt = p.dtime * p.dt_ref
+        want = 0.0
+        t0 = time.perf_counter()
+        for s in range(1, steps + 1):
+            eng.run(s, s)
+            cnt = eng.cell_scalars()["count"].astype(np.float64)
+            want += float(np.sum(0.5 * cnt * (cnt - 1.0) / vol)) * p.Neff * sigma * cr * dt
+            if s % 200 == 0:
+                log(f"[validate box] prec={prec} step {s}/{steps} ({time.perf_counter() - t0:.1f} s)")
+        st = eng.stats()
+        q = eng.download()
+        out["fp64" if prec == D.PREC_FP64 else "mixed"] = {
+            "particles": int(st["particles"]), "dropped": int(st["dropped"]),
+            "energy_rel_drift": (_energy(q) - e0) / e0,
+            "collisions": int(st["collisions"]), "collisions_expected": want,
+            "collision_rate_ratio": st["collisions"] / want,
+            "collision_rate_sigma": math.sqrt(want) / want,
+            "vx_kurtosis": float(np.mean(q.vx ** 4) / np.mean(q.vx ** 2) ** 2),
+        }
+        eng.close()
+    return {"validate": "equilibrium box", "cells": mesh.ncell, "particles": parts.n, "steps": steps, **out}
+
+
+def validate_apollo(steps: int, ppc: float = 100.0, nss: int = 5000, nscheme: int = 2000):
+    """Apollo capsule at full size in FP64, FP64 with another seed (the
+    sampling-noise floor) and mixed: drag coefficient, peak heat flux and the
+    per-field relative L2 differences of the steady-averaged cell fields."""
+    runs = {}
+    for tag, prec, seed in (("fp64", D.PREC_FP64, 20261203), ("fp64_seed2", D.PREC_FP64, 7),
+                            ("mixed", D.PREC_MIXED, 20261203)):
+        mesh, p = make_apollo(ppc, prec == D.PREC_MIXED)
+        p.NSS, p.NSCHEME, p.surface_stats = nss, nscheme, 1
+        parts = D.fill_freestream(mesh, p, ppc=ppc, seed=seed)
+        eng = D.Engine(mesh, p, seed=seed)
+        eng.upload(parts)
+        tuning, s = eng.autotune(1, 2)
+        s += 1
+        t0 = time.perf_counter()
+        while s <= steps:
+            last = min(steps, s + 499)
+            eng.run(s, last)
+            log(f"[validate apollo] {tag} step {last}/{steps} ({time.perf_counter() - t0:.1f} s)")
+            s = last + 1
+        wall = time.perf_counter() - t0
+        fields, cd, qpeak = eng.surface_fields(steps, 3.912)
+        st = eng.stats()
+        runs[tag] = {"macro": eng.macro(D.MACRO_FINAL), "q": fields[:, 3] + fields[:, 4], "normal": fields[:, 5:8],
+                     "uhat": np.array(list(p.u_inf)) / np.linalg.norm(list(p.u_inf)), "area": mesh.arrays()["face_area"],
+                     "cd": cd, "qpeak": qpeak,
+                     "particles": int(st["particles"]), "seconds": wall, "tuning": tuning,
+                     "ps_per_s": st["particle_steps"] / max(wall, 1e-9)}
+        eng.close()
+    a = runs["fp64"]
+    # stagnation region, chosen by geometry only (so that no run's sampling
+    # noise picks the faces): the wall faces whose normal is most aligned
+    # with the free stream, 2% of the wall faces; the wall faces are those
+    # with a normal in the surface output
+    wall = np.linalg.norm(a["normal"], axis=1) > 0.5
+    align = np.where(wall, a["normal"] @ a["uhat"], -np.inf)
+    hot = np.argsort(align)[::-1][: max(1, int(0.02 * wall.sum()))]
+    area = a["area"]
+
+    def stag(r):
+        return float(np.sum(r["q"][hot] * area[hot]) / np.sum(area[hot]))
+
+    def load(r):
+        return float(np.sum(r["q"][wall] * area[wall]))
+    names = ["rho", "ux", "uy", "uz", "T", "sxx", "sxy", "syy", "syz", "szz", "szx", "qx", "qy", "qz", "Tr",
+             "qrx", "qry", "qrz"]
+    rep = {"validate": "apollo", "cells": int(a["macro"].shape[0]), "steps": steps, "ppc": ppc, "NSS": nss,
+           "NSCHEME": nscheme}
+    for tag in ("fp64_seed2", "mixed"):
+        b = runs[tag]
+        fe = {}
+        for f, nm in enumerate(names):
+            den = float(np.linalg.norm(a["macro"][:, f]))
+            if den > 0.0:
+                fe[nm] = float(np.linalg.norm(b["macro"][:, f] - a["macro"][:, f])) / den
+        rep[tag] = {"cd": b["cd"], "cd_rel_diff": (b["cd"] - a["cd"]) / a["cd"],
+                    "qpeak_max_face": b["qpeak"], "qpeak_max_face_rel_diff": (b["qpeak"] - a["qpeak"]) / a["qpeak"],
+                    "q_stagnation": stag(b), "q_stagnation_rel_diff": stag(b) / stag(a) - 1.0,
+                    "heat_load": load(b), "heat_load_rel_diff": load(b) / load(a) - 1.0,
+                    "field_rel_l2_diff": fe, "seconds": b["seconds"], "particle_steps_per_s": b["ps_per_s"],
+                    "tuning": b["tuning"]}
+    rep["stagnation_faces"] = int(len(hot))
+    rep["fp64"] = {"cd": a["cd"], "qpeak_max_face": a["qpeak"], "q_stagnation": stag(a), "heat_load": load(a),
+                   "seconds": a["seconds"], "particle_steps_per_s": a["ps_per_s"], "particles": a["particles"],
+                   "tuning": a["tuning"]}
+    rep["mixed_speedup"] = runs["mixed"]["ps_per_s"] / a["ps_per_s"]
+    return rep
 
 
 def main():
@@ -153,6 +277,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--validate", choices=["box", "apollo"],
+                    help="run a full-size physics validation instead of the benchmark and print its report")
+    ap.add_argument("--validate-steps", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB plain copy kernel (counter calibration, achievable HBM GB/s)")
@@ -163,6 +290,12 @@ def main():
         sys.exit(2)
 
     D.lib()  # fail loudly if the engine library is missing
+    if args.validate == "box":
+        print(json.dumps(validate_box(args.validate_steps or 2000)), flush=True)
+        return
+    if args.validate == "apollo":
+        print(json.dumps(validate_apollo(args.validate_steps or 10000)), flush=True)
+        return
     copy_gbs = None
     if args.copy_probe:
         copy_gbs = D.copy_probe(1 << 30, 5)
@@ -200,9 +333,20 @@ def main():
     nmean = fp64["particle_steps"] / args.steps
     launch_ms = groups[dom] / args.steps
     achieved = per_particle * nmean / (launch_ms / 1e3) / 1e9
+    # HBM bytes per launch of that group, measured with rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE counters on this workload (profiles/traffic.json, calibrated
+    # against a plain copy); null for workloads it does not cover
+    traffic = None
+    tkey = "box" if (args.workload == "box" and args.n == 64 and args.ppc == 50.0) else \
+        ("apollo" if (args.workload == "apollo" and args.ppc == 20.0) else None)
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if tkey and os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(tkey, {}).get(dom, {}).get("traffic_bytes")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "bytes_per_particle": per_particle, "launch_ms": round(launch_ms, 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "bytes_per_particle": per_particle, "launch_ms": round(launch_ms, 4),
+                "alg_bytes_per_launch": round(per_particle * nmean)}
     step_bytes = sum(alg_bytes(k, 64, 32, 80.0 * ntri, nmean_ppc) for k in ("move", "sort", "collide_sample"))
 
     cpu = None

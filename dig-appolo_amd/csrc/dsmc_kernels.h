@@ -62,7 +62,7 @@ struct Patch {
   double R[9];  // rotation built from -n, row major
   double stp1, stp2, snp, fs1, fs2, tN, sqrtT, QA;
   double flux;  // inserted particles per step before the remainder (add_num / Neff)
-  int32_t cell, face, tag, pad;
+  int32_t cell, face, tag, kase;  // kase: the case the patch belongs to
 };
 
 // Live counters and statistics (device memory).
@@ -71,25 +71,36 @@ struct Counters {
   int32_t n_total;   // n + emitted this step
   int32_t n_emit;    // emitted this step
   int32_t overflow;  // capacity exceeded (sticky)
-  uint32_t next_id;
-  int32_t last_step;
+  int32_t last_step;   // engine step being run (set by the step-begin kernel)
   int32_t park_count;  // particles parked at a wall by the move kernel this step
-  int32_t id_base;     // first id of this step's new inflow particles
   int32_t wide_n;      // cells queued for the one-wave-per-cell collide+sample kernel
   int32_t wide_ticket; // next queued cell to hand out in that kernel
   unsigned long long emitted, removed, dropped, wall_hits, crossings, coll_pairs, collisions, particle_steps;
 };
 
-struct KParams {
+// Physics and bookkeeping of one case (one entry per case of a batch; a
+// single-case engine has one).  `ncell` (the accumulator stride) and
+// `max_iter` are the engine's and repeated in every entry so that device code
+// needs only this record.
+struct CaseParams {
   uint64_t key;
   double dtime, Tw, sqrtTw, Neff, dt_ref, coll_prefactor, v_rms, omega, alpha, P_relax, eta;
   double inv_lb;  // 1 / (2.5 - omega)
   double inv_alpha;
   double dr, n_ref;
-  int32_t wall_model, rot, surface, NSS, NSCHEME, Nevery, Nrepeat, ncell;
-  int32_t capacity, npatch, ninlet_cells, max_iter;
+  int32_t wall_model, rot, surface, NSS, NSCHEME, Nevery, Nrepeat, ncell, max_iter;
+  int32_t step0;           // engine steps before the case's step 1 (its step = engine step - step0)
+  int32_t cell0, cell1;    // the case's cells [cell0, cell1)
+  int32_t patch0, patch1;  // the case's inflow patches [patch0, patch1)
+};
+
+// Launch-wide parameters.
+struct KParams {
+  int32_t ncell, capacity, npatch, ninlet_cells, max_iter, ncase;
   int32_t wide_min;    // cells with more particles go one wave per cell (< 0: never)
   int32_t move_chunk;  // particles per move workgroup
+  const CaseParams* __restrict__ cases;
+  const uint16_t* __restrict__ cell_case;  // case of every cell
 };
 
 struct CrossLog {

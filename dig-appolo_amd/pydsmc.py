@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import types
 
 import numpy as np
 
@@ -79,6 +80,11 @@ class Tuning(C.Structure):
                 ("reserved", C.c_int32 * 5)]
 
 
+class CaseInfo(C.Structure):
+    _fields_ = [("ncase", C.c_int32), ("cell_begin", C.c_int32), ("cell_end", C.c_int32),
+                ("face_begin", C.c_int32), ("face_end", C.c_int32), ("particles", C.c_int64)]
+
+
 class CrossEvent(C.Structure):
     _fields_ = [("id", C.c_uint32), ("step", C.c_int32), ("seq", C.c_int32), ("face", C.c_int32),
                 ("tri", C.c_int32), ("kind", C.c_int32)]
@@ -93,6 +99,7 @@ EXPORTED = [
     "dsmc_particle_count", "dsmc_step", "dsmc_run", "dsmc_track", "dsmc_track_log_enable", "dsmc_track_log_get",
     "dsmc_macro_get", "dsmc_statistic_pre", "dsmc_accum_get", "dsmc_cell_scalars_get", "dsmc_surface_get",
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
+    "dsmc_batch_create", "dsmc_case_info", "dsmc_case_surface_fields",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
     "dsmc_device_math_check",
     "dsmc_copy_probe", "dsmc_last_error",
@@ -283,6 +290,13 @@ def fill_freestream(mesh: Mesh, p: Params, ppc: float, temperature: float = 0.0,
     return parts
 
 
+def concat_particles(parts: list) -> Particles:
+    out = Particles(sum(q.n for q in parts))
+    for f in Particles.FIELDS + ("cell", "id"):
+        setattr(out, f, np.ascontiguousarray(np.concatenate([getattr(q, f) for q in parts])))
+    return out
+
+
 class Engine:
     """One DSMC case resident on the current HIP device."""
 
@@ -292,6 +306,61 @@ class Engine:
         self.h = C.c_void_p()
         check(lib().dsmc_engine_create(C.byref(mesh.view), C.byref(p), C.c_uint64(seed), C.c_int64(capacity),
                                        C.byref(self.h)), "engine_create")
+
+    @classmethod
+    def batch(cls, meshes: list, params: list, seeds: list, capacity: int = 0) -> "Engine":
+        """Several cases in one engine (see dsmc_batch_create)."""
+        self = cls.__new__(cls)
+        self.meshes = list(meshes)
+        self.case_params = list(params)
+        self.h = C.c_void_p()
+        k = len(meshes)
+        views = (MeshView * k)(*[m.view for m in meshes])
+        ps = (Params * k)(*params)
+        sd = (C.c_uint64 * k)(*[int(x) for x in seeds])
+        check(lib().dsmc_batch_create(k, views, ps, sd, C.c_int64(capacity), C.byref(self.h)), "batch_create")
+        self.params = params[0]
+        last = self.case_info(k - 1)
+        # batch-wide sizes for the array getters: every case's cell range is
+        # padded to a multiple of 64, so the batch ends at the last case's end
+        # rounded up to 64
+        self.mesh = types.SimpleNamespace(ncell=(last["cell_end"] + 63) // 64 * 64, nface=last["face_end"])
+        return self
+
+    def ncase(self) -> int:
+        i = CaseInfo()
+        check(lib().dsmc_case_info(self.h, 0, C.byref(i)), "case_info")
+        return i.ncase
+
+    def case_info(self, k: int) -> dict:
+        i = CaseInfo()
+        check(lib().dsmc_case_info(self.h, int(k), C.byref(i)), "case_info")
+        return {f: getattr(i, f) for f, _ in CaseInfo._fields_}
+
+    def case_particles(self, parts: Particles, k: int) -> Particles:
+        """The particles of case k from a batch download, cells renumbered to the case's own mesh."""
+        i = self.case_info(k)
+        sel = np.flatnonzero((parts.cell >= i["cell_begin"]) & (parts.cell < i["cell_end"]))
+        q = parts.take(sel)
+        q.cell = (q.cell - i["cell_begin"]).astype(np.int32)
+        return q
+
+    def batch_particles(self, per_case: list) -> Particles:
+        """Concatenates per-case particle sets, cells renumbered to the batch's."""
+        out = []
+        for k, q in enumerate(per_case):
+            r = q.copy()
+            r.cell = (r.cell + self.case_info(k)["cell_begin"]).astype(np.int32)
+            out.append(r)
+        return concat_particles(out)
+
+    def case_surface_fields(self, k: int, istep: int, diameter: float):
+        f = np.zeros((self.mesh.nface, 12))
+        cd = C.c_double()
+        q = C.c_double()
+        check(lib().dsmc_case_surface_fields(self.h, int(k), istep, C.c_double(diameter), _ptr(f, C.c_double),
+                                             C.byref(cd), C.byref(q)), "case_surface_fields")
+        return f, cd.value, q.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -308,6 +308,28 @@ int dsmc_profile_get(dsmc_engine *e, dsmc_kernel_times *t);
  * dsmc_run call, milliseconds. */
 int dsmc_last_run_ms(dsmc_engine *e, double *ms);
 
+/* ---- batches (no reference counterpart: the reference runs one case per
+ *      job).  Several cases — each its own mesh, parameters and seed — run in
+ *      one engine: every kernel launch of a step covers all of them.  A case's
+ *      results are bit-identical to running it alone with the same seed (its
+ *      random streams are keyed by its own cell, patch and particle numbers).
+ *      The batch's cells are the cases' cells in order, each case padded with
+ *      empty cells to a multiple of 64; dsmc_case_info gives the ranges to
+ *      slice cell and face outputs, and particles uploaded to a batch use
+ *      these cell numbers (ids stay per case). */
+int dsmc_batch_create(int32_t ncase, const dsmc_mesh_view *meshes, const dsmc_params *params,
+                      const uint64_t *seeds, int64_t capacity, dsmc_engine **out);
+typedef struct dsmc_case_info_t {
+  int32_t ncase;
+  int32_t cell_begin, cell_end; /* the case's own cells in the batch */
+  int32_t face_begin, face_end; /* the case's faces in the batch */
+  int64_t particles;            /* particles of the case now */
+} dsmc_case_info_t;
+int dsmc_case_info(dsmc_engine *e, int32_t kase, dsmc_case_info_t *info);
+/* dsmc_surface_fields for one case of a batch (istep: the case's step). */
+int dsmc_case_surface_fields(dsmc_engine *e, int32_t kase, int32_t istep, double ref_diameter, double *fields,
+                             double *drag_coefficient, double *peak_heat_flux);
+
 /* Bytes of device memory the engine holds. */
 int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
 
