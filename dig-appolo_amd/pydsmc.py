@@ -77,7 +77,7 @@ TUNE_KEEP = -2147483648
 
 class Tuning(C.Structure):
     _fields_ = [("wide_min", C.c_int32), ("wide_grid", C.c_int32), ("move_chunk", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("graph", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class CaseInfo(C.Structure):
@@ -465,10 +465,11 @@ class Engine:
         return b.value
 
     def set_tuning(self, wide_min: int | None = None, wide_grid: int | None = None,
-                   move_chunk: int | None = None):
+                   move_chunk: int | None = None, graph: int | None = None):
         """Launch tuning; results are bit-identical for every setting."""
         t = Tuning()
-        for f, v in (("wide_min", wide_min), ("wide_grid", wide_grid), ("move_chunk", move_chunk)):
+        for f, v in (("wide_min", wide_min), ("wide_grid", wide_grid), ("move_chunk", move_chunk),
+                     ("graph", graph)):
             setattr(t, f, TUNE_KEEP if v is None else int(v))
         check(lib().dsmc_engine_set_tuning(self.h, C.byref(t)), "set_tuning")
 
@@ -479,9 +480,10 @@ class Engine:
         used = C.c_int32()
         check(lib().dsmc_autotune(self.h, int(first_step), int(steps_per_trial), C.byref(t), C.byref(used)),
               "autotune")
-        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk}, used.value
+        tuned = {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk, "graph": t.graph}
+        return tuned, used.value
 
     def tuning(self) -> dict:
         t = Tuning()
         check(lib().dsmc_engine_get_tuning(self.h, C.byref(t)), "get_tuning")
-        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk}
+        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk, "graph": t.graph}

@@ -343,13 +343,18 @@ int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
  *   move_chunk particles per workgroup of the move kernel, whose lanes take
  *              the chunk's particles one after another (64..1048576,
  *              default 2048)
+ *   graph      1: dsmc_run replays pairs of steps captured as a HIP graph
+ *              (steps numbered on the device; not while profiling or
+ *              logging); 0: every step is launched kernel by kernel
+ *              (default 1)
  * Fields set to DSMC_TUNE_KEEP are left unchanged. */
 #define DSMC_TUNE_KEEP (-2147483647 - 1)
 typedef struct dsmc_tuning {
   int32_t wide_min;
   int32_t wide_grid;
   int32_t move_chunk;
-  int32_t reserved[5];
+  int32_t graph;
+  int32_t reserved[4];
 } dsmc_tuning;
 int dsmc_engine_set_tuning(dsmc_engine *e, const dsmc_tuning *t);
 int dsmc_engine_get_tuning(dsmc_engine *e, dsmc_tuning *t);
