@@ -97,6 +97,7 @@ struct CaseParams {
 // Launch-wide parameters.
 struct KParams {
   int32_t ncell, capacity, npatch, ninlet_cells, max_iter, ncase;
+  int32_t park_cap;  // entries of the wall queue
   int32_t wide_min;    // cells with more particles go one wave per cell (< 0: never)
   int32_t move_chunk;  // particles per move workgroup
   const CaseParams* __restrict__ cases;

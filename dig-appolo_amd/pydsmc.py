@@ -99,7 +99,7 @@ EXPORTED = [
     "dsmc_particle_count", "dsmc_step", "dsmc_run", "dsmc_track", "dsmc_track_log_enable", "dsmc_track_log_get",
     "dsmc_macro_get", "dsmc_statistic_pre", "dsmc_accum_get", "dsmc_cell_scalars_get", "dsmc_surface_get",
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
-    "dsmc_batch_create", "dsmc_case_info", "dsmc_case_surface_fields",
+    "dsmc_batch_create", "dsmc_case_info", "dsmc_case_surface_fields", "dsmc_fill_device",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
     "dsmc_device_math_check",
     "dsmc_copy_probe", "dsmc_last_error",
@@ -353,6 +353,21 @@ class Engine:
             r.cell = (r.cell + self.case_info(k)["cell_begin"]).astype(np.int32)
             out.append(r)
         return concat_particles(out)
+
+    def fill_device(self, ppc, seeds, temperature=None) -> int:
+        """Device-side initial fill (per case: ppc, seed, temperature or None)."""
+        ppc = [float(x) for x in (ppc if hasattr(ppc, "__len__") else [ppc])]
+        seeds = [int(x) for x in (seeds if hasattr(seeds, "__len__") else [seeds])]
+        k = len(ppc)
+        pa = (C.c_double * k)(*ppc)
+        sd = (C.c_uint64 * k)(*seeds)
+        ta = None
+        if temperature is not None:
+            t = [float(x) for x in (temperature if hasattr(temperature, "__len__") else [temperature])]
+            ta = (C.c_double * k)(*t)
+        n = C.c_int64()
+        check(lib().dsmc_fill_device(self.h, pa, ta, sd, C.byref(n)), "fill_device")
+        return n.value
 
     def case_surface_fields(self, k: int, istep: int, diameter: float):
         f = np.zeros((self.mesh.nface, 12))

@@ -214,6 +214,14 @@ void dsmc_engine_destroy(dsmc_engine *e);
  * soa->n smaller than the particle count returns DSMC_ERR_CAPACITY and the
  * needed size in *count. */
 int dsmc_particles_upload(dsmc_engine *e, const dsmc_particles_soa *soa);
+/* Initial fill generated on the device (no reference counterpart; replaces
+ * dsmc_fill_freestream + dsmc_particles_upload for cases too large to fill
+ * on the host): for every case k, bit-identically the particles that
+ * dsmc_fill_freestream(case k's mesh, its params, ppc[k], temperature[k],
+ * seeds[k]) would produce, placed and state reset as an upload does.
+ * temperature may be null (free-stream temperature). */
+int dsmc_fill_device(dsmc_engine *e, const double *ppc, const double *temperature, const uint64_t *seeds,
+                     int64_t *count);
 int dsmc_particles_download(dsmc_engine *e, dsmc_particles_soa *soa, int64_t *count);
 int dsmc_particle_count(dsmc_engine *e, int64_t *count);
 

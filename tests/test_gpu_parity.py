@@ -270,3 +270,29 @@ def test_graph_replay_matches_eager():
         for k in a:
             assert a[k] == b[k], (last, k, a[k], b[k])
     assert np.array_equal(g.macro(D.MACRO_OUTPUT), e.macro(D.MACRO_OUTPUT), equal_nan=True)
+
+
+@pytest.mark.parametrize("prec", [D.PREC_FP64, D.PREC_MIXED])
+def test_device_fill_matches_host_fill(prec):
+    """The device-side fill places bit-identically the particles of the host
+    fill plus upload, for a single engine and for every case of a batch."""
+    cases = [_case("capsule"), _case("box_tets"), _case("capsule_tets")]
+    ppcs, seeds = [8.0, 10.0, 6.0], [101, 102, 103]
+    temps = [0.0, 1.0, 0.0]
+    for mesh, p, _ in cases:
+        p.precision = prec
+    host = []
+    for (mesh, p, _), ppc, sd, T in zip(cases, ppcs, seeds, temps):
+        parts = D.fill_freestream(mesh, p, ppc=ppc, temperature=T, seed=sd)
+        a = D.Engine(mesh, p, seed=1)
+        a.upload(parts)
+        b = D.Engine(mesh, p, seed=1)
+        n = b.fill_device(ppc, sd, T)
+        assert n == parts.n
+        _same_particles(b.download(), a.download())
+        host.append(a.download())
+    bat = D.Engine.batch([c[0] for c in cases], [c[1] for c in cases], [1, 2, 3])
+    bat.fill_device(ppcs, seeds, temps)
+    allp = bat.download()
+    for k, ref in enumerate(host):
+        _same_particles(bat.case_particles(allp, k), ref)
