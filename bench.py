@@ -266,13 +266,131 @@ def validate_apollo(steps: int, ppc: float = 100.0, nss: int = 5000, nscheme: in
     return rep
 
 
+def run_sweep(args):
+    """Batched sweep: `--sweep-cases` seeded capsule cases (seeds 0..N-1) in
+    one batch engine, mixed precision, device fill, autotuned, steps replayed
+    as HIP graphs; then every case alone, one after another, for the same
+    steps (the unbatched schedule), for comparison."""
+    import sweep as SW
+    t0 = time.perf_counter()
+    specs = [SW.case_spec(i, args.sweep_scale) for i in range(args.sweep_cases)]
+    built = [SW.build_case(sp, D.PREC_MIXED, steps=2000) for sp in specs]
+    log(f"[sweep] {len(specs)} cases built in {time.perf_counter() - t0:.1f} s")
+    seeds = [1000 + sp["seed"] for sp in specs]
+    want = sum(sp["ppc"] * m.ncell for sp, (m, _) in zip(specs, built))
+    cap = int(1.25 * want) + 2_000_000 * len(specs)
+    eng = D.Engine.batch([m for m, _ in built], [p for _, p in built], seeds, capacity=cap)
+    n = eng.fill_device([sp["ppc"] for sp in specs], seeds)
+    log(f"[sweep] batch: {eng.mesh.ncell} cells, {n} particles, {eng.footprint() / 1e9:.1f} GB "
+        f"({time.perf_counter() - t0:.1f} s)")
+    tuning, used = eng.autotune(1, 2)
+    s = used + 1
+    if args.warmup:
+        eng.run(s, s + args.warmup - 1)
+        s += args.warmup
+    st0 = eng.stats()
+    ms = eng.run(s, s + args.steps - 1)
+    st1 = eng.stats()
+    ps = st1["particle_steps"] - st0["particle_steps"]
+    batched = ps / (ms / 1e3)
+    log(f"[sweep] batched: {batched:.4g} particle-steps/s, {ms / args.steps:.2f} ms/step, tuning {tuning}")
+    eng.close()
+    seq_ps, seq_ms = 0, 0.0
+    if not args.no_sequential:
+        for k, ((m, p), sp) in enumerate(zip(built, specs)):
+            e1 = D.Engine(m, p, seed=seeds[k], capacity=int(1.25 * sp["ppc"] * m.ncell) + 2_000_000)
+            e1.fill_device(sp["ppc"], seeds[k])
+            e1.set_tuning(**{kk: v for kk, v in tuning.items() if kk != "graph"})
+            e1.run(1, 1 + args.warmup)
+            a = e1.stats()["particle_steps"]
+            seq_ms += e1.run(2 + args.warmup, 1 + args.warmup + args.steps)
+            seq_ps += e1.stats()["particle_steps"] - a
+            e1.close()
+            log(f"[sweep] case {k} alone done ({time.perf_counter() - t0:.1f} s)")
+        log(f"[sweep] one case after another: {seq_ps / (seq_ms / 1e3):.4g} particle-steps/s")
+    return {
+        "metric": METRIC, "value": round(batched, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 velocities / f64 positions",
+        "data": "synthetic (seeded capsule cases, device fill)",
+        "config": {"workload": f"batched sweep of {len(specs)} capsule tet cases (seeds 0..{len(specs) - 1}, "
+                               f"cell scale {args.sweep_scale:g}), Ma U[3,10], Kn logU[0.005,0.1], AoA U[0,40], "
+                               "20-100 ppc, mixed precision, one engine, graph replay",
+                   "cells": int(sum(m.ncell for m, _ in built)), "particles": int(n), "parallelism": "single GPU"},
+        "sequential_value": None if not seq_ms else round(seq_ps / (seq_ms / 1e3), 1),
+        "batched_speedup": None if not seq_ms else round(batched / (seq_ps / (seq_ms / 1e3)), 3),
+        "tuning": tuning, "tune_steps": used,
+    }
+
+
+def run_serve(args):
+    """Serving queue: `--serve-requests` seeded capsule cases arrive as a
+    Poisson stream (`--serve-rate` per second), each a `--serve-steps` solve
+    in mixed precision; the server admits them by HBM footprint and runs the
+    admitted ones concurrently.  Latency = finish - arrival."""
+    import sweep as SW
+    t_set = time.perf_counter()
+    specs = [SW.case_spec(i, args.serve_scale) for i in range(args.serve_requests)]
+    built = [SW.build_case(sp, D.PREC_MIXED, steps=args.serve_steps) for sp in specs]
+    log(f"[serve] {len(specs)} requests prepared in {time.perf_counter() - t_set:.1f} s")
+    rng = np.random.default_rng(20261203)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(specs)))
+    arrivals -= arrivals[0]
+    srv = D.Server(budget_bytes=0, chunk_steps=args.serve_chunk)
+    t0 = time.perf_counter()
+    nxt, ids, last_log = 0, [], t0
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < len(specs) and arrivals[nxt] <= now:
+            m, p = built[nxt]
+            ids.append(srv.submit(m, p, seed=1000 + nxt, ppc=specs[nxt]["ppc"], steps=args.serve_steps))
+            nxt += 1
+        running, queued, finished = srv.pump()
+        if nxt == len(specs) and finished == len(specs):
+            break
+        if time.perf_counter() - last_log > 15.0:
+            last_log = time.perf_counter()
+            log(f"[serve] t={now:.1f} s: {nxt} arrived, {running} running, {queued} queued, {finished} finished")
+        time.sleep(0.0005)
+    makespan = time.perf_counter() - t0
+    infos = [srv.request(i) for i in ids]
+    srv.close()
+    failed = [i["error"] for i in infos if i["state"] == D.REQ_FAILED]
+    lat = np.array([i["t_finish"] - i["t_submit"] for i in infos])
+    wait = np.array([i["t_admit"] - i["t_submit"] for i in infos])
+    total_ps = float(sum(i["particle_steps"] for i in infos))
+    p50, p99 = (float(x) for x in np.percentile(lat, [50, 99]))
+    log(f"[serve] {len(infos)} requests in {makespan:.1f} s: p50 {p50:.2f} s, p99 {p99:.2f} s, "
+        f"{total_ps / makespan:.4g} particle-steps/s, {len(failed)} failed")
+    return {
+        "metric": METRIC, "value": round(total_ps / makespan, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.serve_steps, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 velocities / f64 positions", "data": "synthetic (seeded capsule cases)",
+        "config": {"workload": f"serving queue: {len(specs)} capsule requests (cell scale {args.serve_scale:g}), "
+                               f"Poisson arrivals {args.serve_rate:g}/s, {args.serve_steps}-step solves, "
+                               f"chunks of {args.serve_chunk} steps, HBM-footprint admission, mixed precision",
+                   "particles_initial": int(sum(i["particles"] for i in infos)), "parallelism": "single GPU"},
+        "latency_s": {"p50": round(p50, 3), "p99": round(p99, 3), "max": round(float(lat.max()), 3),
+                      "mean_queue_wait": round(float(wait.mean()), 3)},
+        "makespan_s": round(makespan, 2), "failed": failed,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64, help="box resolution (hexes per side)")
-    ap.add_argument("--workload", choices=["box", "apollo"], default="box")
+    ap.add_argument("--workload", choices=["box", "apollo", "sweep", "serve"], default="box")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="mean arrivals per second")
+    ap.add_argument("--serve-steps", type=int, default=1000)
+    ap.add_argument("--serve-chunk", type=int, default=50)
+    ap.add_argument("--serve-scale", type=float, default=0.1, help="multiplies every request's cell count")
+    ap.add_argument("--sweep-cases", type=int, default=32)
+    ap.add_argument("--sweep-scale", type=float, default=1.0, help="multiplies every sweep case's cell count")
+    ap.add_argument("--no-sequential", action="store_true", help="sweep: skip the one-case-at-a-time comparison")
     ap.add_argument("--ppc", type=float, default=50.0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -295,6 +413,12 @@ def main():
         return
     if args.validate == "apollo":
         print(json.dumps(validate_apollo(args.validate_steps or 10000)), flush=True)
+        return
+    if args.workload == "sweep":
+        print(json.dumps(run_sweep(args)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args)), flush=True)
         return
     copy_gbs = None
     if args.copy_probe:

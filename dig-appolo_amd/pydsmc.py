@@ -85,6 +85,16 @@ class CaseInfo(C.Structure):
                 ("face_begin", C.c_int32), ("face_end", C.c_int32), ("particles", C.c_int64)]
 
 
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("steps", C.c_int32), ("steps_done", C.c_int32), ("pad", C.c_int32),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_finish", C.c_double),
+                ("estimate", C.c_int64), ("footprint", C.c_int64), ("particles", C.c_int64),
+                ("particle_steps", C.c_int64), ("error", C.c_char * 256)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_RELEASED = 0, 1, 2, 3, 4
+
+
 class CrossEvent(C.Structure):
     _fields_ = [("id", C.c_uint32), ("step", C.c_int32), ("seq", C.c_int32), ("face", C.c_int32),
                 ("tri", C.c_int32), ("kind", C.c_int32)]
@@ -100,6 +110,8 @@ EXPORTED = [
     "dsmc_macro_get", "dsmc_statistic_pre", "dsmc_accum_get", "dsmc_cell_scalars_get", "dsmc_surface_get",
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
     "dsmc_batch_create", "dsmc_case_info", "dsmc_case_surface_fields", "dsmc_fill_device",
+    "dsmc_run_async", "dsmc_run_query", "dsmc_engine_estimate", "dsmc_server_create", "dsmc_server_destroy",
+    "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_request", "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
     "dsmc_device_math_check",
     "dsmc_copy_probe", "dsmc_last_error",
@@ -117,10 +129,11 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.dsmc_last_error.restype = C.c_char_p
         for name in EXPORTED:
-            if name not in ("dsmc_last_error", "dsmc_mesh_destroy", "dsmc_engine_destroy"):
+            if name not in ("dsmc_last_error", "dsmc_mesh_destroy", "dsmc_engine_destroy", "dsmc_server_destroy"):
                 getattr(L, name).restype = C.c_int
         L.dsmc_mesh_destroy.restype = None
         L.dsmc_engine_destroy.restype = None
+        L.dsmc_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -474,6 +487,14 @@ class Engine:
         check(lib().dsmc_profile_get(self.h, C.byref(t)), "profile_get")
         return {g: (t.ms[i], t.calls[i]) for i, g in enumerate(KERNEL_GROUPS)}
 
+    @staticmethod
+    def estimate(mesh: Mesh, p: Params, capacity: int) -> int:
+        """Upper bound of the device bytes an engine for this case allocates."""
+        b = C.c_int64()
+        check(lib().dsmc_engine_estimate(C.byref(mesh.view), C.byref(p), C.c_int64(capacity), C.byref(b)),
+              "estimate")
+        return b.value
+
     def footprint(self) -> int:
         b = C.c_int64()
         check(lib().dsmc_engine_footprint(self.h, C.byref(b)), "footprint")
@@ -502,3 +523,66 @@ class Engine:
         t = Tuning()
         check(lib().dsmc_engine_get_tuning(self.h, C.byref(t)), "get_tuning")
         return {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk, "graph": t.graph}
+
+
+class _Borrowed(Engine):
+    """An engine owned by a server (not destroyed by this object)."""
+
+    def close(self):
+        self.h = None
+
+    def __del__(self):
+        pass
+
+
+class Server:
+    """Serving loop with continuous batching (see dsmc_server_create)."""
+
+    def __init__(self, budget_bytes: int = 0, chunk_steps: int = 50, auto_release: bool = True):
+        self.h = C.c_void_p()
+        check(lib().dsmc_server_create(C.c_int64(budget_bytes), int(chunk_steps), int(auto_release),
+                                       C.byref(self.h)), "server_create")
+        self.meshes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dsmc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, mesh: Mesh, p: Params, seed: int, ppc: float, steps: int, capacity: int = 0) -> int:
+        """Queues a request; the server takes over the mesh."""
+        rid = C.c_int64()
+        check(lib().dsmc_server_submit(self.h, mesh.h, C.byref(p), C.c_uint64(seed), C.c_double(ppc), int(steps),
+                                       C.c_int64(capacity), C.byref(rid)), "server_submit")
+        self.meshes[rid.value] = (mesh.ncell, mesh.nface)
+        mesh.h = None  # owned by the server now (its arrays go with it)
+        return rid.value
+
+    def pump(self) -> tuple[int, int, int]:
+        r, q, f = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().dsmc_server_pump(self.h, C.byref(r), C.byref(q), C.byref(f)), "server_pump")
+        return r.value, q.value, f.value
+
+    def request(self, rid: int) -> dict:
+        i = RequestInfo()
+        check(lib().dsmc_server_request(self.h, int(rid), C.byref(i)), "server_request")
+        d = {f: getattr(i, f) for f, _ in RequestInfo._fields_ if f not in ("pad", "error")}
+        d["error"] = i.error.decode(errors="replace")
+        return d
+
+    def engine(self, rid: int, params: Params) -> Engine:
+        """The engine of a finished, unreleased request (borrowed)."""
+        h = C.c_void_p()
+        check(lib().dsmc_server_engine(self.h, int(rid), C.byref(h)), "server_engine")
+        e = _Borrowed.__new__(_Borrowed)
+        e.h = h
+        e.params = params
+        nc, nf = self.meshes[rid]
+        e.mesh = types.SimpleNamespace(ncell=nc, nface=nf)
+        return e
+
+    def release(self, rid: int):
+        check(lib().dsmc_server_release(self.h, int(rid)), "server_release")

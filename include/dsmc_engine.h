@@ -233,6 +233,11 @@ int dsmc_step(dsmc_engine *e, int32_t istep, uint32_t stage_mask);
 /* Run steps first..last (inclusive) with DSMC_STAGE_ALL, asynchronously on
  * the engine's stream; synchronises at the end. */
 int dsmc_run(dsmc_engine *e, int32_t first, int32_t last);
+/* Asynchronous dsmc_run (no reference counterpart; for callers that drive
+ * several engines at once): enqueues the steps and returns; dsmc_run_query
+ * sets *done once they have completed (then reporting their errors). */
+int dsmc_run_async(dsmc_engine *e, int32_t first, int32_t last);
+int dsmc_run_query(dsmc_engine *e, int32_t *done);
 /* Move only (the minimum slice): advection of every particle for one dtime
  * (src/ProcessDSMC.cpp:1946-2017) followed by the regroup. */
 int dsmc_track(dsmc_engine *e, int32_t istep);
@@ -337,6 +342,43 @@ int dsmc_case_info(dsmc_engine *e, int32_t kase, dsmc_case_info_t *info);
 /* dsmc_surface_fields for one case of a batch (istep: the case's step). */
 int dsmc_case_surface_fields(dsmc_engine *e, int32_t kase, int32_t istep, double ref_diameter, double *fields,
                              double *drag_coefficient, double *peak_heat_flux);
+
+/* Upper bound of the device bytes dsmc_engine_create(mesh, p, seed,
+ * capacity) allocates (capacity > 0). */
+int dsmc_engine_estimate(const dsmc_mesh_view *mesh, const dsmc_params *p, int64_t capacity, int64_t *bytes);
+
+/* ---- serving (no reference counterpart) ------------------------------------
+ * A queue of case requests served with continuous batching: requests are
+ * admitted in arrival order while their estimated footprint fits the HBM
+ * budget (<= 0: 85% of the free device memory); each admitted request runs
+ * in its own engine on its own stream, advanced `chunk_steps` at a time
+ * with asynchronous runs, so admitted requests overlap on the GPU; a finished
+ * request frees its memory (auto_release) and queued ones are admitted.
+ * Results are bit-identical to the same case run alone.  dsmc_server_pump
+ * never blocks: call it repeatedly until every request has finished.
+ * dsmc_server_submit takes ownership of `mesh`; capacity <= 0 picks
+ * 1.25 x ppc x cells + 2M.  Times are seconds since dsmc_server_create. */
+typedef struct dsmc_server dsmc_server;
+#define DSMC_REQ_QUEUED 0
+#define DSMC_REQ_RUNNING 1
+#define DSMC_REQ_DONE 2
+#define DSMC_REQ_FAILED 3
+#define DSMC_REQ_RELEASED 4
+typedef struct dsmc_request_info {
+  int32_t state, steps, steps_done, pad;
+  double t_submit, t_admit, t_finish;
+  int64_t estimate, footprint, particles, particle_steps;
+  char error[256];
+} dsmc_request_info;
+int dsmc_server_create(int64_t hbm_budget_bytes, int32_t chunk_steps, int32_t auto_release, dsmc_server **out);
+void dsmc_server_destroy(dsmc_server *s);
+int dsmc_server_submit(dsmc_server *s, dsmc_mesh *mesh, const dsmc_params *p, uint64_t seed, double ppc,
+                       int32_t steps, int64_t capacity, int64_t *request_id);
+int dsmc_server_pump(dsmc_server *s, int32_t *running, int32_t *queued, int32_t *finished);
+int dsmc_server_request(dsmc_server *s, int64_t id, dsmc_request_info *info);
+/* The engine of a finished (not released) request, to read its results. */
+int dsmc_server_engine(dsmc_server *s, int64_t id, dsmc_engine **engine);
+int dsmc_server_release(dsmc_server *s, int64_t id);
 
 /* Bytes of device memory the engine holds. */
 int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);

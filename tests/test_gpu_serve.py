@@ -1,0 +1,65 @@
+"""Batched sweep and serving on the GPU: the generated cases, the engine
+footprint estimate, and the server's continuous batching (each request's
+result bit-identical alone and in the mix)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pydsmc as D
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dig-appolo_amd"))
+import sweep as SW  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(g, c):
+    assert g.n == c.n
+    for f in ("x", "y", "z", "vx", "vy", "vz", "ir", "cell", "id"):
+        assert np.array_equal(getattr(g, f), getattr(c, f)), f
+
+
+def test_footprint_estimate_bounds_the_engine():
+    for seed in (0, 3):
+        m, p = SW.build_case(SW.case_spec(seed, 0.02), D.PREC_MIXED, steps=20)
+        cap = int(1.25 * p.Npinitial * m.ncell) + 2_000_000
+        e = D.Engine(m, p, seed=1, capacity=cap)
+        est = e.estimate(m, p, cap)
+        got = e.footprint()
+        print(f"[estimate] seed={seed} cells={m.ncell} est={est} got={got}", flush=True)
+        assert got <= est <= 1.15 * got + (80 << 20)
+        e.close()
+
+
+def test_served_requests_match_alone():
+    """Three requests served concurrently (chunks of 6 steps, so their runs
+    interleave on the device) end bit-identical to each case run alone."""
+    steps, seeds = 20, [501, 502, 503]
+    specs = [SW.case_spec(i, 0.02) for i in (1, 2, 3)]
+    srv = D.Server(budget_bytes=0, chunk_steps=6, auto_release=False)
+    ids = []
+    for sp, sd in zip(specs, seeds):
+        m, p = SW.build_case(sp, D.PREC_MIXED, steps=steps)
+        ids.append(srv.submit(m, p, seed=sd, ppc=sp["ppc"], steps=steps))
+    while True:
+        running, queued, finished = srv.pump()
+        if finished == len(ids):
+            break
+    print(f"[serve] {[srv.request(i)['state'] for i in ids]}", flush=True)
+    for rid, sp, sd in zip(ids, specs, seeds):
+        info = srv.request(rid)
+        assert info["state"] == D.REQ_DONE, info["error"]
+        m, p = SW.build_case(sp, D.PREC_MIXED, steps=steps)
+        alone = D.Engine(m, p, seed=sd)
+        alone.fill_device(sp["ppc"], sd)
+        alone.run(1, steps)
+        served = srv.engine(rid, p)
+        _same(served.download(), alone.download())
+        for fam in (0, 1, 2):
+            assert np.array_equal(served.accum(fam), alone.accum(fam)), fam
+        assert info["particle_steps"] == alone.stats()["particle_steps"]
+        alone.close()
+        srv.release(rid)
+    srv.close()
