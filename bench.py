@@ -71,10 +71,16 @@ def make_apollo(ppc: float, mixed: bool):
     return mesh, p
 
 
-def run_engine(mesh, p, parts, steps, warmup, seed=20261203, autotune=True):
+def run_engine(mesh, p, parts, steps, warmup, seed=20261203, autotune=True, tuning=None):
     eng = D.Engine(mesh, p, seed=seed)
     eng.upload(parts)
-    tuning, tune_steps = eng.tuning(), 0
+    tune_steps = 0
+    if tuning:
+        # launch settings given by the caller (a profiler run replaying the
+        # settings a timed run picked)
+        eng.set_tuning(**tuning)
+        autotune = False
+    tuning = eng.tuning()
     if autotune:
         # set-up: the autotuner times live steps (results do not depend on the
         # tuning it picks), then the case restarts from the same particles
@@ -395,6 +401,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", type=json.loads, default=None,
+                    help='launch settings instead of the autotuner, e.g. \'{"move_chunk": 4096, "wide_min": 64}\'')
     ap.add_argument("--validate", choices=["box", "apollo"],
                     help="run a full-size physics validation instead of the benchmark and print its report")
     ap.add_argument("--validate-steps", type=int, default=0)
@@ -436,14 +444,15 @@ def main():
         parts = D.fill_freestream(mesh, p, ppc=args.ppc, temperature=1.0, seed=20261203)
         ntri, label = 4.0, f"box {args.n}^3x6 tets"
     log(f"[bench] {label}: {mesh.ncell} cells, {parts.n} particles (set-up {time.perf_counter() - t0:.1f} s)")
-    fp64 = run_engine(mesh, p, parts, args.steps, args.warmup, autotune=not args.no_autotune)
+    fp64 = run_engine(mesh, p, parts, args.steps, args.warmup, autotune=not args.no_autotune, tuning=args.tuning)
     log(f"[bench] fp64: {fp64['value']:.4g} particle-steps/s, {fp64['ms_per_step']:.2f} ms/step")
 
     mixed = None
     if not args.no_mixed:
         pm = p.copy()
         pm.precision = D.PREC_MIXED
-        mixed = run_engine(mesh, pm, parts, args.steps, args.warmup, autotune=not args.no_autotune)
+        mixed = run_engine(mesh, pm, parts, args.steps, args.warmup, autotune=not args.no_autotune,
+                           tuning=args.tuning)
         log(f"[bench] mixed: {mixed['value']:.4g} particle-steps/s, {mixed['ms_per_step']:.2f} ms/step")
 
     # roofline of the dominant kernel group, from the HIP events around each
