@@ -66,6 +66,15 @@ struct Patch {
 };
 
 // Live counters and statistics (device memory).
+// Per-step event tallies.  Workgroup b adds into slot b % TALLY_SLOTS, each
+// slot on its own 128-byte line: device atomics on one line serialise
+// (about 12 ns apiece on MI355X), so tens of thousands of workgroups adding
+// into one counter would queue behind each other.  Readers sum the slots.
+constexpr int TALLY_SLOTS = 64;
+struct alignas(128) TallySlot {
+  unsigned long long removed, dropped, wall_hits, crossings, coll_pairs, collisions;
+};
+
 struct Counters {
   int32_t n;         // particles grouped by cell at step start
   int32_t n_total;   // n + emitted this step
@@ -74,8 +83,8 @@ struct Counters {
   int32_t last_step;   // engine step being run (set by the step-begin kernel)
   int32_t park_count;  // particles parked at a wall by the move kernel this step
   int32_t wide_n;      // cells queued for the one-wave-per-cell collide+sample kernel
-  int32_t wide_ticket; // next queued cell to hand out in that kernel
-  unsigned long long emitted, removed, dropped, wall_hits, crossings, coll_pairs, collisions, particle_steps;
+  unsigned long long emitted, particle_steps;
+  TallySlot tally[TALLY_SLOTS];
 };
 
 // Physics and bookkeeping of one case (one entry per case of a batch; a

@@ -66,6 +66,14 @@ def _case(kind, seed=3):
         p.Npinitial = 100
         D.derive(mesh, p)
         parts = D.fill_freestream(mesh, p, ppc=100.0, temperature=1.0, seed=seed)
+    elif kind == "box_crowded":
+        # ~600 particles per cell: more than a wave's LDS stage holds, so the
+        # one-wave-per-cell kernel works on the cell in HBM
+        mesh = D.Mesh.box_tets(2)
+        p = D.params_box(0.05)
+        p.Npinitial = 600
+        D.derive(mesh, p)
+        parts = D.fill_freestream(mesh, p, ppc=600.0, temperature=1.0, seed=seed)
     elif kind == "box_hex":
         mesh = D.Mesh.box_hexes(6)
         p = D.params_box(0.05)
@@ -113,7 +121,8 @@ def test_track_parity(kind, move_chunk):
 # every cell one thread per cell
 @pytest.mark.parametrize("kind,wide_min", [("box_tets", None), ("capsule", None), ("capsule_tets", None),
                                            ("box_dense", None), ("box_dense", -1), ("capsule", 0),
-                                           ("box_dense", 0), ("box_dense", 20)])
+                                           ("box_dense", 0), ("box_dense", 20), ("box_crowded", None),
+                                           ("box_crowded", -1)])
 def test_full_step_parity(kind, wide_min):
     mesh, p, parts = _case(kind)
     print(f"[full_step_parity] {kind} wide_min={wide_min} cells={mesh.ncell} particles={parts.n}", flush=True)
@@ -145,6 +154,33 @@ def test_full_step_parity(kind, wide_min):
             assert st_g[k] == st_c[k], (k, st_g[k], st_c[k])
     sg, sc = eng.surface(), orc.surface()
     np.testing.assert_allclose(sg, sc, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [D.PREC_FP64, D.PREC_MIXED])
+@pytest.mark.parametrize("kind", ["box_dense", "box_crowded"])
+def test_wide_matches_thread_path(kind, prec):
+    """Every cell one wave (staged in LDS for box_dense, worked on in HBM for
+    box_crowded) against every cell one thread: the same particles, cell
+    scalars and accumulators bit for bit, in FP64 and in the mixed form
+    (which has no oracle)."""
+    mesh, p, parts = _case(kind)
+    p.precision = prec
+    print(f"[wide_vs_thread] {kind} prec={prec} cells={mesh.ncell} particles={parts.n}", flush=True)
+    out = []
+    for wm in (0, -1):
+        eng = D.Engine(mesh, p, seed=31)
+        eng.set_tuning(wide_min=wm)
+        eng.upload(parts)
+        eng.run(1, 6)
+        out.append((eng.download(), eng.cell_scalars(), [eng.accum(f) for f in (0, 1, 2)], eng.stats()))
+        eng.close()
+    (qa, ca, aa, sa), (qb, cb, ab, sb) = out
+    _same_particles(qa, qb)
+    for k in ca:
+        assert np.array_equal(ca[k], cb[k]), k
+    for fa, fb in zip(aa, ab):
+        assert np.array_equal(fa, fb)
+    assert sa["collisions"] == sb["collisions"] > 0
 
 
 def test_stagewise_matches_full_step():
