@@ -342,7 +342,7 @@ def run_serve(args):
     rng = np.random.default_rng(20261203)
     arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(specs)))
     arrivals -= arrivals[0]
-    srv = D.Server(budget_bytes=0, chunk_steps=args.serve_chunk)
+    srv = D.Server(budget_bytes=0, chunk_steps=args.serve_chunk, max_running=args.serve_max_running)
     t0 = time.perf_counter()
     nxt, ids, last_log = 0, [], t0
     while True:
@@ -374,7 +374,10 @@ def run_serve(args):
         "vs_baseline": None, "dtype": "f32 velocities / f64 positions", "data": "synthetic (seeded capsule cases)",
         "config": {"workload": f"serving queue: {len(specs)} capsule requests (cell scale {args.serve_scale:g}), "
                                f"Poisson arrivals {args.serve_rate:g}/s, {args.serve_steps}-step solves, "
-                               f"chunks of {args.serve_chunk} steps, HBM-footprint admission, mixed precision",
+                               f"chunks of {args.serve_chunk} steps, HBM-footprint admission"
+                               + (f", at most {args.serve_max_running} running" if args.serve_max_running else "")
+                               + f", {os.environ.get('GPU_MAX_HW_QUEUES', '4')} hardware queues"
+                               + ", mixed precision",
                    "particles_initial": int(sum(i["particles"] for i in infos)), "parallelism": "single GPU"},
         "latency_s": {"p50": round(p50, 3), "p99": round(p99, 3), "max": round(float(lat.max()), 3),
                       "mean_queue_wait": round(float(wait.mean()), 3)},
@@ -393,6 +396,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="mean arrivals per second")
     ap.add_argument("--serve-steps", type=int, default=1000)
     ap.add_argument("--serve-chunk", type=int, default=50)
+    ap.add_argument("--serve-max-running", type=int, default=8, help="requests running at once (0: no limit)")
+    ap.add_argument("--serve-hw-queues", type=int, default=16,
+                    help="HIP hardware queues of the serving process (sets GPU_MAX_HW_QUEUES; 0 keeps the environment's)")
     ap.add_argument("--serve-scale", type=float, default=0.1, help="multiplies every request's cell count")
     ap.add_argument("--sweep-cases", type=int, default=32)
     ap.add_argument("--sweep-scale", type=float, default=1.0, help="multiplies every sweep case's cell count")
@@ -415,6 +421,11 @@ def main():
               "(use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serve" and args.serve_hw_queues > 0:
+        # concurrently running engines each need a hardware queue of their
+        # own (HIP's default is 4 per process); read at HIP initialisation,
+        # which has not happened yet in this process
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.serve_hw_queues, 32))
     D.lib()  # fail loudly if the engine library is missing
     if args.validate == "box":
         print(json.dumps(validate_box(args.validate_steps or 2000)), flush=True)

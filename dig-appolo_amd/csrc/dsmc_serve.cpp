@@ -55,6 +55,7 @@ struct dsmc_server {
   int64_t in_use = 0;
   int32_t chunk = 50;
   int32_t auto_release = 1;
+  int32_t max_running = 0;  // 0: no limit
   std::vector<Request> reqs;
   std::deque<size_t> queue;  // indices of queued requests, arrival order
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -183,10 +184,15 @@ int dsmc_server_pump(dsmc_server* s, int32_t* running, int32_t* queued, int32_t*
       fail(s, r);
     }
   }
-  // admission in arrival order while the next request fits
+  // admission in arrival order while the next request fits (and the
+  // running-request cap allows)
+  int32_t nrun = 0;
+  for (const Request& r : s->reqs) nrun += (r.state == REQ_RUNNING);
   while (!s->queue.empty()) {
     Request& r = s->reqs[s->queue.front()];
     if (s->in_use + r.estimate > s->budget) break;
+    if (s->max_running > 0 && nrun >= s->max_running) break;
+    ++nrun;
     s->queue.pop_front();
     s->in_use += r.estimate;
     r.reserved = true;
@@ -213,6 +219,12 @@ int dsmc_server_pump(dsmc_server* s, int32_t* running, int32_t* queued, int32_t*
   if (running) *running = nr;
   if (queued) *queued = nq;
   if (finished) *finished = nf;
+  return DSMC_OK;
+}
+
+int dsmc_server_set_max_running(dsmc_server* s, int32_t max_running) {
+  if (!s || max_running < 0) return set_error(DSMC_ERR_ARG, "dsmc_server_set_max_running: bad arguments");
+  s->max_running = max_running;
   return DSMC_OK;
 }
 

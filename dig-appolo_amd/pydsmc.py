@@ -111,7 +111,8 @@ EXPORTED = [
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
     "dsmc_batch_create", "dsmc_case_info", "dsmc_case_surface_fields", "dsmc_fill_device",
     "dsmc_run_async", "dsmc_run_query", "dsmc_engine_estimate", "dsmc_server_create", "dsmc_server_destroy",
-    "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_request", "dsmc_server_engine", "dsmc_server_release",
+    "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_request",
+    "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
     "dsmc_device_math_check",
     "dsmc_copy_probe", "dsmc_last_error",
@@ -538,11 +539,14 @@ class _Borrowed(Engine):
 class Server:
     """Serving loop with continuous batching (see dsmc_server_create)."""
 
-    def __init__(self, budget_bytes: int = 0, chunk_steps: int = 50, auto_release: bool = True):
+    def __init__(self, budget_bytes: int = 0, chunk_steps: int = 50, auto_release: bool = True,
+                 max_running: int = 0):
         self.h = C.c_void_p()
         check(lib().dsmc_server_create(C.c_int64(budget_bytes), int(chunk_steps), int(auto_release),
                                        C.byref(self.h)), "server_create")
         self.meshes = {}
+        if max_running:
+            check(lib().dsmc_server_set_max_running(self.h, int(max_running)), "server_set_max_running")
 
     def close(self):
         if getattr(self, "h", None):

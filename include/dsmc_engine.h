@@ -375,6 +375,11 @@ void dsmc_server_destroy(dsmc_server *s);
 int dsmc_server_submit(dsmc_server *s, dsmc_mesh *mesh, const dsmc_params *p, uint64_t seed, double ppc,
                        int32_t steps, int64_t capacity, int64_t *request_id);
 int dsmc_server_pump(dsmc_server *s, int32_t *running, int32_t *queued, int32_t *finished);
+/* At most max_running requests run at once (0: no limit, the default); the
+ * rest wait in the queue even when their footprint fits.  Engines that run
+ * at the same time share the device's hardware queues, so a cap near their
+ * number keeps each engine's kernels from waiting behind the others'. */
+int dsmc_server_set_max_running(dsmc_server *s, int32_t max_running);
 int dsmc_server_request(dsmc_server *s, int64_t id, dsmc_request_info *info);
 /* The engine of a finished (not released) request, to read its results. */
 int dsmc_server_engine(dsmc_server *s, int64_t id, dsmc_engine **engine);

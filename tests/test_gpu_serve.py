@@ -63,3 +63,27 @@ def test_served_requests_match_alone():
         alone.close()
         srv.release(rid)
     srv.close()
+
+
+def test_running_cap_holds_requests_in_the_queue():
+    """With at most one request running, three submitted requests run one
+    after another in arrival order (each admitted only after the previous
+    one finished), and all finish."""
+    steps = 8
+    specs = [SW.case_spec(i, 0.02) for i in (4, 5, 6)]
+    srv = D.Server(budget_bytes=0, chunk_steps=4, max_running=1)
+    ids = []
+    for k, sp in enumerate(specs):
+        m, p = SW.build_case(sp, D.PREC_MIXED, steps=steps)
+        ids.append(srv.submit(m, p, seed=600 + k, ppc=sp["ppc"], steps=steps))
+    while True:
+        running, queued, finished = srv.pump()
+        assert running <= 1
+        if finished == len(ids):
+            break
+    infos = [srv.request(i) for i in ids]
+    print(f"[serve cap] admits {[round(i['t_admit'], 3) for i in infos]}", flush=True)
+    assert all(i["state"] == D.REQ_RELEASED for i in infos), [i["error"] for i in infos]
+    for a, b in zip(infos, infos[1:]):
+        assert b["t_admit"] >= a["t_finish"]
+    srv.close()
