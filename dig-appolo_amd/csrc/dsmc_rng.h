@@ -29,6 +29,7 @@ enum RngPurpose : uint32_t {
   RNG_COLLIDE = 4u,     // NTC pair selection and collision draws, per cell
   RNG_INIT = 5u,        // initial particle fill, per cell
   RNG_INIT_COUNT = 6u,  // random rounding of the initial per-cell count
+  RNG_GSIS = 7u,        // continuum coupling: count rounding (sub 0), cell update (sub 1)
 };
 
 struct Philox4 {

@@ -114,7 +114,7 @@ EXPORTED = [
     "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_request",
     "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
-    "dsmc_device_math_check",
+    "dsmc_device_math_check", "dsmc_ns_apply",
     "dsmc_copy_probe", "dsmc_last_error",
 ]
 
@@ -509,6 +509,18 @@ class Engine:
                      ("graph", graph)):
             setattr(t, f, TUNE_KEEP if v is None else int(v))
         check(lib().dsmc_engine_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def ns_apply(self, istep: int, ns: np.ndarray, accepted: np.ndarray | None = None) -> int:
+        """Continuum coupling (molecular_velocity_change): ns is ncell x 6
+        (rho/n_ref, u (3), T, Tr); returns the first coupled cell with a NaN
+        NS state (left unchanged with every later cell), or -1."""
+        a = np.ascontiguousarray(ns, dtype=np.float64).reshape(-1, 6)
+        acc = None if accepted is None else np.ascontiguousarray(accepted, dtype=np.uint8)
+        nan_cell = C.c_int32()
+        check(lib().dsmc_ns_apply(self.h, int(istep), a.ctypes.data_as(C.POINTER(C.c_double)),
+                                  None if acc is None else acc.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                  C.byref(nan_cell)), "ns_apply")
+        return nan_cell.value
 
     def autotune(self, first_step: int, steps_per_trial: int = 2) -> tuple[dict, int]:
         """Tunes the launch parameters on live steps first_step, ...; returns

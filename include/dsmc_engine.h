@@ -385,6 +385,27 @@ int dsmc_server_request(dsmc_server *s, int64_t id, dsmc_request_info *info);
 int dsmc_server_engine(dsmc_server *s, int64_t id, dsmc_engine **engine);
 int dsmc_server_release(dsmc_server *s, int64_t id);
 
+/* ---- continuum coupling (GSIS) ---------------------------------------------
+ * Device-side molecular_velocity_change (src/ProcessGSIS.cpp:2239-2345) with
+ * replication (:2368-2410), MaxwellianSample (:2424-2446), randomlocation
+ * (:2459-2474) and deletion (:2487-2528): replaces the reference's update of
+ * the particle buckets from the NS solution.  ns: ncell x 6 per cell = density
+ * / n_ref, mean velocity (3, v_rms units), T and T_r (T_ref units), as
+ * nsresult.  accepted: ncell flags (the DSMC2NS acceptance gate), or NULL for
+ * every cell.  For each coupled cell, in cell order:
+ *   - the count becomes Iround(rho V n_ref / Neff) by appending copies of
+ *     random particles at new random points of the cell (Maxwellian samples
+ *     of the NS state if the cell was empty and T >= 0), or by deleting random
+ *     particles (swap with the last);
+ *   - velocities become k v + b and Ir becomes kr Ir with k = sqrt(T/T_cell),
+ *     b = u - k u_cell, kr = T_r/T_r,cell when the cell's T, T_r and the NS T
+ *     are positive (else unchanged).
+ * A NaN in the first four NS fields of a coupled cell stops the update there
+ * (that cell and later cells stay as they are, as in the reference);
+ * *nan_cell receives that cell or -1.  istep (> 0) keys the random streams.
+ * New particles take the next ids of their case.  Call between steps. */
+int dsmc_ns_apply(dsmc_engine *e, int32_t istep, const double *ns, const uint8_t *accepted, int32_t *nan_cell);
+
 /* Bytes of device memory the engine holds. */
 int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
 

@@ -36,6 +36,8 @@ int oracle_step(dsmc_oracle *o, int32_t istep, uint32_t stage_mask);
 int oracle_log_enable(dsmc_oracle *o, int32_t on);
 int64_t oracle_log_get(dsmc_oracle *o, dsmc_cross_event *out, int64_t cap);
 int oracle_macro_get(dsmc_oracle *o, int32_t which, double *out);
+int oracle_ns_apply(dsmc_oracle *o, int32_t call_step, const double *ns, const uint8_t *accepted,
+                    int32_t *nan_cell);
 int oracle_statistic_pre(dsmc_oracle *o);
 int oracle_accum_get(dsmc_oracle *o, int32_t family, double *out);
 int oracle_cell_scalars(dsmc_oracle *o, int32_t *count, double *crmax, double *rem_coll, double *rem_emit,

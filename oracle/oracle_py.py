@@ -120,6 +120,17 @@ class Oracle:
         lib().oracle_macro_get(self.h, which, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
 
+    def ns_apply(self, istep: int, ns: np.ndarray, accepted: np.ndarray | None = None) -> int:
+        a = np.ascontiguousarray(ns, dtype=np.float64).reshape(-1, 6)
+        acc = None if accepted is None else np.ascontiguousarray(accepted, dtype=np.uint8)
+        nan_cell = C.c_int32()
+        rc = lib().oracle_ns_apply(self.h, int(istep), a.ctypes.data_as(C.POINTER(C.c_double)),
+                                   None if acc is None else acc.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                   C.byref(nan_cell))
+        if rc:
+            raise RuntimeError(f"oracle_ns_apply failed ({rc})")
+        return nan_cell.value
+
     def statistic_pre(self):
         lib().oracle_statistic_pre(self.h)
 

@@ -332,3 +332,79 @@ def test_device_fill_matches_host_fill(prec):
     allp = bat.download()
     for k, ref in enumerate(host):
         _same_particles(bat.case_particles(allp, k), ref)
+
+
+def _ns_fields(m, p, counts, seed=17):
+    """NS fields for a coupling test: density scaled by 0.6-1.5 per cell,
+    random drift, T and T_r in [0.5, 2], 80% of cells coupled, a NaN density
+    in one coupled cell at 70% of the cells."""
+    rng = np.random.default_rng(seed)
+    nc = m.ncell
+    vol = m.arrays()["cell_volume"]
+    ns = np.empty((nc, 6))
+    ns[:, 0] = counts * p.Neff / (vol * p.n_ref) * rng.uniform(0.6, 1.5, nc)
+    ns[:, 1:4] = rng.normal(0.0, 0.5, (nc, 3))
+    ns[:, 4] = rng.uniform(0.5, 2.0, nc)
+    ns[:, 5] = rng.uniform(0.5, 2.0, nc)
+    acc = (rng.uniform(size=nc) < 0.8).astype(np.uint8)
+    stop = int(np.flatnonzero(acc)[int(0.7 * acc.sum())])
+    ns[stop, 0] = np.nan
+    return ns, acc, stop
+
+
+@pytest.mark.parametrize("kind", ["capsule", "capsule_tets", "box_tets"])
+def test_ns_apply_parity(kind):
+    """Continuum coupling on the device (dsmc_ns_apply: molecular_velocity_change
+    with replication, deletion and Maxwellian refill) against the oracle: the
+    same particles bit for bit after the update, and after two more steps."""
+    mesh, p, parts = _case(kind)
+    print(f"[ns_apply_parity] {kind} cells={mesh.ncell} particles={parts.n}", flush=True)
+    eng = D.Engine(mesh, p, seed=21)
+    eng.upload(parts)
+    orc = O.Oracle(mesh, p, seed=21)
+    orc.set_particles(parts)
+    for s in (1, 2):
+        eng.step(s)
+        orc.step(s)
+    counts = eng.cell_scalars()["count"].astype(np.float64)
+    ns, acc, stop = _ns_fields(mesh, p, counts)
+    assert eng.ns_apply(3, ns, acc) == stop
+    assert orc.ns_apply(3, ns, acc) == stop
+    _same_particles(eng.download(), orc.particles())
+    assert not np.array_equal(eng.cell_scalars()["count"], counts)
+    for s in (3, 4):
+        eng.step(s)
+        orc.step(s)
+    _same_particles(eng.download(), orc.particles())
+    for k in ("particles", "collisions", "crossings"):
+        assert eng.stats()[k] == orc.stats()[k], k
+    # every cell coupled, no NaN: the whole mesh is updated
+    ns2, _, _ = _ns_fields(mesh, p, eng.cell_scalars()["count"].astype(np.float64), seed=5)
+    ns2[:, 0] = np.abs(np.nan_to_num(ns2[:, 0], nan=1.0))
+    assert eng.ns_apply(5, ns2) == -1 and orc.ns_apply(5, ns2) == -1
+    _same_particles(eng.download(), orc.particles())
+
+
+def test_ns_apply_mixed_counts_match_fp64():
+    """The mixed engine's coupling makes the same count, deletion and
+    replication choices as the FP64 engine (they do not depend on the stored
+    velocity precision): same cells, ids and positions; velocities agree to
+    FP32 rounding."""
+    mesh, p, parts = _case("capsule")
+    out = []
+    for prec in (D.PREC_FP64, D.PREC_MIXED):
+        q = p.copy()
+        q.precision = prec
+        eng = D.Engine(mesh, q, seed=21)
+        eng.upload(parts)
+        counts = eng.cell_scalars()["count"].astype(np.float64)
+        ns, acc, stop = _ns_fields(mesh, q, counts)
+        assert eng.ns_apply(1, ns, acc) == stop
+        out.append(eng.download())
+        eng.close()
+    a, b = out
+    assert a.n == b.n
+    for f in ("cell", "id", "x", "y", "z"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    for f in ("vx", "vy", "vz"):
+        assert np.allclose(getattr(a, f), getattr(b, f), rtol=1e-5, atol=1e-5), f

@@ -319,3 +319,132 @@ def test_params_and_mesh_errors_are_reported():
     p.Kn = -1.0
     with pytest.raises(D.DsmcError):
         D.derive(m, p)
+
+
+def _ns_case(kind):
+    """A small case after two steps, and NS fields for it: density scaled by
+    0.6-1.5 per cell, random drift, T and T_r in [0.5, 2], 80% of cells
+    coupled, a NaN density in one coupled cell at 70% of the cells."""
+    if kind == "capsule":
+        m = D.Mesh.capsule(a=4, c=8, layers=4, diameter=3.912)
+        p = D.params_apollo(monatomic=False)  # diatomic: T_r > 0, so the shift applies
+        p.Npinitial = 12
+        D.derive(m, p)
+        parts = D.fill_freestream(m, p, ppc=12.0, seed=4)
+    else:
+        m, p, parts = _box(n=3, ppc=20.0, T=1.0, seed=6)
+    o = O.Oracle(m, p, seed=13)
+    o.set_particles(parts)
+    o.step(1)
+    o.step(2)
+    rng = np.random.default_rng(17)
+    nc = m.ncell
+    vol = m.arrays()["cell_volume"]
+    cnt = np.bincount(o.particles().cell, minlength=nc).astype(np.float64)
+    ns = np.empty((nc, 6))
+    ns[:, 0] = cnt * p.Neff / (vol * p.n_ref) * rng.uniform(0.6, 1.5, nc)
+    ns[:, 1:4] = rng.normal(0.0, 0.5, (nc, 3))
+    ns[:, 4] = rng.uniform(0.5, 2.0, nc)
+    ns[:, 5] = rng.uniform(0.5, 2.0, nc)
+    acc = (rng.uniform(size=nc) < 0.8).astype(np.uint8)
+    stop = int(np.flatnonzero(acc)[int(0.7 * acc.sum())])
+    ns[stop, 0] = np.nan
+    return m, p, o, ns, acc, stop
+
+
+def _by_cell(q):
+    order = np.lexsort((np.arange(q.n), q.cell))
+    return {f: getattr(q, f)[order] for f in ("cell", "id", "x", "y", "z", "vx", "vy", "vz", "ir")}
+
+
+@pytest.mark.parametrize("kind", ["capsule", "box"])
+def test_ns_apply_counts_moments_and_untouched_cells(kind):
+    """Continuum coupling on the oracle (molecular_velocity_change,
+    src/ProcessGSIS.cpp:2239-2345): a coupled cell before the NaN cell ends
+    with Iround(rho V n_ref / Neff) particles and, where the shift applies,
+    exactly the NS mean velocity, temperature and rotational temperature;
+    every other cell keeps its particles bit for bit; new particles get fresh
+    ids and points inside their cell's bounding box."""
+    m, p, o, ns, acc, stop = _ns_case(kind)
+    before = _by_cell(o.particles())
+    maxid = int(before["id"].max())
+    assert o.ns_apply(3, ns, acc) == stop
+    after = _by_cell(o.particles())
+    nc = m.ncell
+    a = m.arrays()
+    vol = a["cell_volume"]
+    nb = np.bincount(before["cell"], minlength=nc)
+    na = np.bincount(after["cell"], minlength=nc)
+    sb = np.concatenate([[0], np.cumsum(nb)])
+    sa = np.concatenate([[0], np.cumsum(na)])
+    fn, fc = a["face_nodes"], a["face_cells"]
+    checked = 0
+    for c in range(nc):
+        B = {f: v[sb[c]:sb[c + 1]] for f, v in before.items()}
+        A = {f: v[sa[c]:sa[c + 1]] for f, v in after.items()}
+        if c >= stop or not acc[c]:
+            for f in B:
+                assert np.array_equal(A[f], B[f]), (c, f)
+            continue
+        x = ns[c, 0] * vol[c] * p.n_ref / p.Neff
+        assert na[c] in (math.floor(x), math.floor(x) + 1), (c, na[c], x)
+        new = A["id"] > maxid
+        assert new.sum() == max(0, na[c] - nb[c])
+        old_ids = set(B["id"].tolist())
+        assert all(i in old_ids for i in A["id"][~new])
+        if new.any():
+            faces = np.flatnonzero((fc[:, 0] == c) | (fc[:, 1] == c))
+            nodes = a["node_xyz"][np.unique(fn[faces][fn[faces] >= 0])]
+            lo, hi = nodes.min(0) - 1e-12, nodes.max(0) + 1e-12
+            pts = np.stack([A["x"][new], A["y"][new], A["z"][new]], 1)
+            assert np.all(pts >= lo) and np.all(pts <= hi), c
+        if p.dr == 0.0:
+            # monatomic: T_r of the cell is not positive, so no shift
+            # (src/ProcessGSIS.cpp:2304-2320): kept particles keep velocities
+            vb = dict(zip(B["id"].tolist(), B["vx"].tolist()))
+            assert all(vb[i] == vx for i, vx in zip(A["id"][~new].tolist(), A["vx"][~new].tolist()))
+            checked += 1
+        elif na[c] >= 2:
+            v = np.stack([A["vx"], A["vy"], A["vz"]], 1)
+            u = v.mean(0)
+            T = 2.0 / (3.0 * na[c]) * (np.sum(v * v) - na[c] * np.dot(u, u))
+            Tr = 4.0 / p.dr / na[c] * A["ir"].sum()
+            if T > 0 and Tr > 0:
+                assert np.allclose(u, ns[c, 1:4], rtol=0, atol=1e-12), (c, u, ns[c, 1:4])
+                assert abs(T - ns[c, 4]) < 1e-11 * ns[c, 4], (c, T, ns[c, 4])
+                assert abs(Tr - ns[c, 5]) < 1e-11 * ns[c, 5], (c, Tr, ns[c, 5])
+                checked += 1
+    assert checked > 10
+    ids = after["id"]
+    assert len(np.unique(ids)) == len(ids)
+
+
+def test_ns_apply_refills_an_empty_cell_with_the_ns_maxwellian():
+    """A coupled cell emptied by a zero NS density is refilled from the NS
+    state (MaxwellianSample, src/ProcessGSIS.cpp:2424-2446): with many
+    particles its drift, temperature and rotational temperature follow the NS
+    values statistically before the exact shift, and exactly after it."""
+    m, p, parts = _box(n=2, ppc=20.0, T=1.0, seed=8)
+    p2 = p.copy()
+    p2.dr = 2.0  # rotational energy present, so the shift applies
+    o = O.Oracle(m, p2, seed=3)
+    o.set_particles(parts)
+    nc = m.ncell
+    vol = m.arrays()["cell_volume"]
+    ns = np.zeros((nc, 6))
+    ns[:, 4] = ns[:, 5] = 1.0
+    acc = np.zeros(nc, np.uint8)
+    acc[0] = 1
+    o.ns_apply(1, ns, acc)  # cell 0 emptied
+    assert np.sum(o.particles().cell == 0) == 0
+    ns[0] = [4000.0 * p.Neff / (vol[0] * p.n_ref), 0.3, -0.2, 0.1, 1.5, 0.8]
+    o.ns_apply(2, ns, acc)
+    q = o.particles()
+    sel = q.cell == 0
+    assert sel.sum() in (3999, 4000, 4001)
+    v = np.stack([q.vx[sel], q.vy[sel], q.vz[sel]], 1)
+    n = sel.sum()
+    u = v.mean(0)
+    T = 2.0 / (3.0 * n) * (np.sum(v * v) - n * np.dot(u, u))
+    assert np.allclose(u, [0.3, -0.2, 0.1], atol=1e-12) and abs(T - 1.5) < 1e-11
+    assert abs(4.0 / p2.dr / n * q.ir[sel].sum() - 0.8) < 1e-11
