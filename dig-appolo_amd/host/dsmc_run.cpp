@@ -78,10 +78,18 @@ int main(int argc, char** argv) {
     if (output_step(istep)) {
       die(dsmc_macro_get(eng, DSMC_MACRO_OUTPUT, macro.data()), "macro");
       if (!outdir.empty()) {
-        const std::string fn = outdir + "/DSMC_iter" + std::to_string(istep) + ".bin";
-        if (FILE* f = std::fopen(fn.c_str(), "wb")) {
-          std::fwrite(macro.data(), sizeof(double), macro.size(), f);
-          std::fclose(f);
+        // out2dat's file (src/ProcessDSMC.cpp:3421-3424): Fluent cell fields
+        char fn[256];
+        std::snprintf(fn, sizeof(fn), "%s/DSMCKn%.1f_iter%d_CELL%d.dat", outdir.c_str(), p.Kn, istep, view.ncell);
+        die(dsmc_write_fluent_cells(fn, &p, view.ncell, macro.data(), 2, 1), "cell fields file");
+        if (surface && istep > p.NSS) {
+          // outBoundaryStressHeat's file (src/ProcessDSMC.cpp:3618-3621)
+          std::vector<double> wall((size_t)view.nface * DSMC_SURF_WIDTH);
+          double cd = 0.0, qpeak = 0.0;
+          die(dsmc_surface_fields(eng, istep, p.L0, wall.data(), &cd, &qpeak), "surface fields");
+          std::snprintf(fn, sizeof(fn), "%s/BoundaryStressHeatFluentKn%.1f_iter%d.dat", outdir.c_str(), p.Kn, istep);
+          die(dsmc_write_fluent_walls(fn, &view, wall.data(), 7, 1), "wall fields file");
+          std::printf("step %d: C_D = %.6g, peak q = %.6g\n", istep, cd, qpeak);
         }
       }
     }

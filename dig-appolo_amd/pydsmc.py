@@ -114,7 +114,7 @@ EXPORTED = [
     "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_request",
     "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
-    "dsmc_device_math_check", "dsmc_ns_apply",
+    "dsmc_device_math_check", "dsmc_ns_apply", "dsmc_write_fluent_cells", "dsmc_write_fluent_walls",
     "dsmc_copy_probe", "dsmc_last_error",
 ]
 
@@ -602,3 +602,18 @@ class Server:
 
     def release(self, rid: int):
         check(lib().dsmc_server_release(self.h, int(rid)), "server_release")
+
+
+def write_fluent_cells(path: str, p: Params, w: np.ndarray, zone_id: int = 2, first_id: int = 1):
+    """Fluent cell fields file (out2Fluent_dsmc) of an ncell x 18 macro array."""
+    a = np.ascontiguousarray(w, dtype=np.float64).reshape(-1, 18)
+    check(lib().dsmc_write_fluent_cells(path.encode(), C.byref(p), a.shape[0],
+                                        a.ctypes.data_as(C.POINTER(C.c_double)), int(zone_id), int(first_id)),
+          "write_fluent_cells")
+
+
+def write_fluent_walls(path: str, mesh: Mesh, fields: np.ndarray, zone_id: int = 7, first_id: int = 1):
+    """Fluent wall fields file (outBoundaryStressHeat) of dsmc_surface_fields' nface x 12 output."""
+    a = np.ascontiguousarray(fields, dtype=np.float64).reshape(-1, 12)
+    check(lib().dsmc_write_fluent_walls(path.encode(), C.byref(mesh.view), a.ctypes.data_as(C.POINTER(C.c_double)),
+                                        int(zone_id), int(first_id)), "write_fluent_walls")

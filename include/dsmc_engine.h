@@ -385,6 +385,22 @@ int dsmc_server_request(dsmc_server *s, int64_t id, dsmc_request_info *info);
 int dsmc_server_engine(dsmc_server *s, int64_t id, dsmc_engine **engine);
 int dsmc_server_release(dsmc_server *s, int64_t id);
 
+/* ---- Fluent-format output files -----------------------------------------------
+ * The cell fields file (meshImport::out2Fluent_dsmc, src/meshImport.cpp:
+ * 1107-1182; the reference calls it from out2dat, src/ProcessDSMC.cpp:3421-
+ * 3424, with the gathered ncell x 18 MacroIndex array `w`): records 700
+ * density, 701 velocity, 702 T and T_r, 710 stress, 711 and 712 heat fluxes,
+ * scaled to free-stream units, for cell zone `zone_id` with Fluent ids
+ * first_id .. first_id + ncell - 1 (the Apollo case: zone 2, first 1). */
+int dsmc_write_fluent_cells(const char *path, const dsmc_params *p, int32_t ncell, const double *w,
+                            int32_t zone_id, int32_t first_id);
+/* The wall fields file (outBoundaryStressHeat's writer, src/ProcessDSMC.cpp:
+ * 3590-3652) from dsmc_surface_fields' output: one block per run of
+ * consecutive wall faces (zones zone_id, zone_id + 1, ...; Fluent face ids =
+ * face index + first_id). */
+int dsmc_write_fluent_walls(const char *path, const dsmc_mesh_view *mesh, const double *fields, int32_t zone_id,
+                            int32_t first_id);
+
 /* ---- continuum coupling (GSIS) ---------------------------------------------
  * Device-side molecular_velocity_change (src/ProcessGSIS.cpp:2239-2345) with
  * replication (:2368-2410), MaxwellianSample (:2424-2446), randomlocation
