@@ -115,6 +115,7 @@ EXPORTED = [
     "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
     "dsmc_device_math_check", "dsmc_ns_apply", "dsmc_write_fluent_cells", "dsmc_write_fluent_walls",
+    "dsmc_mesh_read_cas", "dsmc_mesh_write_cas",
     "dsmc_copy_probe", "dsmc_last_error",
 ]
 
@@ -200,6 +201,16 @@ class Mesh:
         check(lib().dsmc_mesh_capsule(a, c, layers, C.c_double(diameter), int(tets), C.c_double(jitter),
                                       C.c_uint64(seed), C.byref(h)), "capsule")
         return Mesh(h)
+
+    @staticmethod
+    def read_cas(path: str, scale: float = 1.0) -> "Mesh":
+        """Fluent ASCII case mesh (readCAS)."""
+        h = C.c_void_p()
+        check(lib().dsmc_mesh_read_cas(path.encode(), C.c_double(scale), C.byref(h)), "mesh_read_cas")
+        return Mesh(h)
+
+    def write_cas(self, path: str):
+        check(lib().dsmc_mesh_write_cas(C.byref(self.view), path.encode()), "mesh_write_cas")
 
     @staticmethod
     def build(node_xyz, face_nodes, face_cells, face_type, face_tag, ncell: int) -> "Mesh":

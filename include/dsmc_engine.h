@@ -100,6 +100,14 @@ typedef struct dsmc_mesh dsmc_mesh;
 int dsmc_mesh_build(int32_t ncell, int32_t nface, int32_t nnode, const double *node_xyz,
                     const int32_t *face_nodes, const int32_t *face_cells,
                     const int32_t *face_type, const int32_t *face_tag, dsmc_mesh **out);
+/* Read a Fluent ASCII case mesh (readCAS, src/meshImport.cpp:177-576): nodes
+ * (times `scale`, the reference's ScaleFactor), cells and faces; each face
+ * zone's id becomes its faces' tag (3 interface, 4 inlet, 5 outlet, 6 wall);
+ * then the geometry as dsmc_mesh_build.  Triangle and quad faces only. */
+int dsmc_mesh_read_cas(const char *path, double scale, dsmc_mesh **out);
+/* Write a mesh in the same format: one node zone, one cell zone (2) and one
+ * face zone per run of consecutive faces with the same tag (zone id = tag). */
+int dsmc_mesh_write_cas(const dsmc_mesh_view *mesh, const char *path);
 /* Unit cube [0,1]^3 of n^3 hexes, each split into 6 tets (Freudenthal), every
  * boundary face tagged `wall_tag`. */
 int dsmc_mesh_box_tets(int32_t n, int32_t wall_tag, dsmc_mesh **out);
