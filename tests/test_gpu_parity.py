@@ -74,6 +74,14 @@ def _case(kind, seed=3):
         p.Npinitial = 600
         D.derive(mesh, p)
         parts = D.fill_freestream(mesh, p, ppc=600.0, temperature=1.0, seed=seed)
+    elif kind == "box_packed":
+        # ~2000 particles per cell: more than the wide kernel's per-particle
+        # conflict words cover, so its batches compare candidates lane by lane
+        mesh = D.Mesh.box_tets(1)
+        p = D.params_box(0.05)
+        p.Npinitial = 2000
+        D.derive(mesh, p)
+        parts = D.fill_freestream(mesh, p, ppc=2000.0, temperature=1.0, seed=seed)
     elif kind == "box_hex":
         mesh = D.Mesh.box_hexes(6)
         p = D.params_box(0.05)
@@ -122,7 +130,7 @@ def test_track_parity(kind, move_chunk):
 @pytest.mark.parametrize("kind,wide_min", [("box_tets", None), ("capsule", None), ("capsule_tets", None),
                                            ("box_dense", None), ("box_dense", -1), ("capsule", 0),
                                            ("box_dense", 0), ("box_dense", 20), ("box_crowded", None),
-                                           ("box_crowded", -1)])
+                                           ("box_crowded", -1), ("box_packed", 0)])
 def test_full_step_parity(kind, wide_min):
     mesh, p, parts = _case(kind)
     print(f"[full_step_parity] {kind} wide_min={wide_min} cells={mesh.ncell} particles={parts.n}", flush=True)
@@ -157,12 +165,13 @@ def test_full_step_parity(kind, wide_min):
 
 
 @pytest.mark.parametrize("prec", [D.PREC_FP64, D.PREC_MIXED])
-@pytest.mark.parametrize("kind", ["box_dense", "box_crowded"])
+@pytest.mark.parametrize("kind", ["box_dense", "box_crowded", "box_packed"])
 def test_wide_matches_thread_path(kind, prec):
     """Every cell one wave (staged in LDS for box_dense, worked on in HBM for
-    box_crowded) against every cell one thread: the same particles, cell
-    scalars and accumulators bit for bit, in FP64 and in the mixed form
-    (which has no oracle)."""
+    box_crowded and box_packed, with per-particle conflict words in LDS up to
+    box_crowded's size and lane-by-lane conflict tests for box_packed) against
+    every cell one thread: the same particles, cell scalars and accumulators
+    bit for bit, in FP64 and in the mixed form (which has no oracle)."""
     mesh, p, parts = _case(kind)
     p.precision = prec
     print(f"[wide_vs_thread] {kind} prec={prec} cells={mesh.ncell} particles={parts.n}", flush=True)
