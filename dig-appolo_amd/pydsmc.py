@@ -114,7 +114,7 @@ EXPORTED = [
     "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_request",
     "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
-    "dsmc_device_math_check", "dsmc_ns_apply", "dsmc_write_fluent_cells", "dsmc_write_fluent_walls",
+    "dsmc_device_math_check", "dsmc_device_accept_check", "dsmc_ns_apply", "dsmc_write_fluent_cells", "dsmc_write_fluent_walls",
     "dsmc_mesh_read_cas", "dsmc_mesh_write_cas",
     "dsmc_copy_probe", "dsmc_last_error",
 ]
@@ -159,6 +159,17 @@ def device_math(u: np.ndarray) -> np.ndarray:
     out = np.zeros((len(u), 6))
     check(lib().dsmc_device_math_check(_ptr(u, C.c_double), C.c_int64(len(u)), _ptr(out, C.c_double)),
           "device_math_check")
+    return out
+
+
+def device_accept(x: np.ndarray, pref: float, eta: float, crm: float) -> np.ndarray:
+    """The collision kernels' FP32 estimate of the NTC acceptance ratio (0
+    where unused) and the deterministic ratio, for speeds x: (n, 2)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((len(x), 2))
+    check(lib().dsmc_device_accept_check(_ptr(x, C.c_double), C.c_int64(len(x)), C.c_double(pref),
+                                         C.c_double(eta), C.c_double(crm), _ptr(out, C.c_double)),
+          "device_accept_check")
     return out
 
 

@@ -474,6 +474,13 @@ int dsmc_autotune(dsmc_engine *e, int32_t first_step, int32_t steps_per_trial, d
  * check that device and host arithmetic agree bit for bit.  No counterpart
  * in the reference. */
 int dsmc_device_math_check(const double *u, int64_t n, double *out);
+/* Diagnostic: the collision kernels' FP32 estimate of the NTC acceptance
+ * ratio coll_diam(x, pref, eta) / crm (0 where it is not used) and the
+ * deterministic ratio itself, for n speeds x (out: n x 2).  The kernels
+ * decide u < ratio from the estimate only when u is outside a relative band
+ * of 2^-11 around it; this checks the estimate's error against that band.
+ * No counterpart in the reference. */
+int dsmc_device_accept_check(const double *x, int64_t n, double pref, double eta, double crm, double *out);
 /* Diagnostic: device-to-device copy of `bytes` with a plain streaming kernel
  * (k_copy_probe), `reps` times; returns the achieved GB/s (read + write).
  * Used to calibrate HBM traffic counters and the achievable bandwidth.  No

@@ -25,6 +25,29 @@ def _same_particles(g, c):
     assert np.array_equal(g.id, c.id)
 
 
+def test_acceptance_estimate_within_its_band():
+    """The collision kernels decide u < c(cr v_rms)/consant_crm from an FP32
+    estimate when u is outside a relative band of 2^-11 around it.  That is
+    exact only if the estimate's error stays inside the band: checked here
+    (to a quarter of the band) over speeds spanning the guarded range, for
+    the Apollo gas's eta and for eta up to 0.5."""
+    rng = np.random.default_rng(7)
+    x = np.concatenate([2.0 ** rng.uniform(-60, 60, 200000), rng.uniform(1, 5000, 200000), [1.0, 2.0 ** -60, 2.0 ** 60]])
+    p = D.params_apollo(monatomic=True)
+    D.derive(D.Mesh.capsule(a=4, c=8, layers=3, diameter=3.912), p)
+    for eta in (p.eta, 0.05, 0.5, -0.2):
+        crm = 0.7 * x[200000:].max() ** (1 - 2 * eta) * p.coll_prefactor
+        out = D.device_accept(x, p.coll_prefactor, eta, crm)
+        est, ratio = out[:, 0], out[:, 1]
+        used = est > 0
+        assert used[200000:].all(), eta
+        rel = np.abs(est[used] / ratio[used] - 1.0)
+        print(f"[accept_estimate] eta={eta:.3f} used={used.sum()} max rel err={rel.max():.3e}", flush=True)
+        assert rel.max() < 2.0 ** -13, (eta, rel.max())
+    # hard spheres never use the estimate
+    assert not (D.device_accept(x[:1000], p.coll_prefactor, 0.0, 1.0)[:, 0] > 0).any()
+
+
 def _sorted_log(ev):
     if len(ev) == 0:
         return ev
@@ -74,6 +97,15 @@ def _case(kind, seed=3):
         p.Npinitial = 600
         D.derive(mesh, p)
         parts = D.fill_freestream(mesh, p, ppc=600.0, temperature=1.0, seed=seed)
+    elif kind == "box_vhs":
+        # variable hard spheres (omega 0.74) with many candidates per cell:
+        # the acceptance test's FP32 estimate and its exact fallback
+        mesh = D.Mesh.box_tets(3)
+        p = D.params_box(0.05)
+        p.omega = 0.74
+        p.Npinitial = 100
+        D.derive(mesh, p)
+        parts = D.fill_freestream(mesh, p, ppc=100.0, temperature=1.0, seed=seed)
     elif kind == "box_packed":
         # ~2000 particles per cell: more than the wide kernel's per-particle
         # conflict words cover, so its batches compare candidates lane by lane
@@ -130,7 +162,8 @@ def test_track_parity(kind, move_chunk):
 @pytest.mark.parametrize("kind,wide_min", [("box_tets", None), ("capsule", None), ("capsule_tets", None),
                                            ("box_dense", None), ("box_dense", -1), ("capsule", 0),
                                            ("box_dense", 0), ("box_dense", 20), ("box_crowded", None),
-                                           ("box_crowded", -1), ("box_packed", 0)])
+                                           ("box_crowded", -1), ("box_packed", 0),
+                                           ("box_vhs", -1), ("box_vhs", 0)])
 def test_full_step_parity(kind, wide_min):
     mesh, p, parts = _case(kind)
     print(f"[full_step_parity] {kind} wide_min={wide_min} cells={mesh.ncell} particles={parts.n}", flush=True)
