@@ -197,6 +197,44 @@ def test_full_step_parity(kind, wide_min):
     np.testing.assert_allclose(sg, sc, rtol=1e-12, atol=1e-12)
 
 
+def test_empty_start_fills_from_the_inlet():
+    """An empty domain at the start: the inflow alone fills the capsule, and
+    every particle, accumulator and count matches the oracle's."""
+    mesh, p, parts = _case("capsule")
+    empty = parts.take(np.arange(0))
+    print(f"[empty_start] capsule cells={mesh.ncell}", flush=True)
+    eng = D.Engine(mesh, p, seed=21)
+    eng.upload(empty)
+    orc = O.Oracle(mesh, p, seed=21)
+    orc.set_particles(empty)
+    for s in range(1, 6):
+        eng.step(s)
+        orc.step(s)
+    g = eng.download()
+    assert g.n > 0
+    _same_particles(g, orc.particles())
+    for fam in (0, 1, 2):
+        assert np.array_equal(eng.accum(fam), orc.accum(fam)), fam
+    st_g, st_c = eng.stats(), orc.stats()
+    for k in ("emitted", "removed", "dropped", "coll_pairs", "collisions"):
+        assert st_g[k] == st_c[k], (k, st_g[k], st_c[k])
+
+
+def test_capacity_overflow_is_an_error():
+    """Inflow beyond the engine's capacity ends the step with a capacity
+    error: the emission is cut at the capacity, so no kernel writes past it,
+    and the engine can still be read and destroyed."""
+    mesh, p, parts = _case("capsule")
+    print(f"[capacity_overflow] capsule cells={mesh.ncell} particles={parts.n} capacity={parts.n + 5}", flush=True)
+    eng = D.Engine(mesh, p, seed=21, capacity=parts.n + 5)
+    eng.upload(parts)
+    with pytest.raises(D.DsmcError, match="capacity"):
+        for s in range(1, 4):
+            eng.step(s)
+    assert eng.download().n <= parts.n + 5
+    eng.close()
+
+
 @pytest.mark.parametrize("prec", [D.PREC_FP64, D.PREC_MIXED])
 @pytest.mark.parametrize("kind", ["box_dense", "box_crowded", "box_packed"])
 def test_wide_matches_thread_path(kind, prec):
