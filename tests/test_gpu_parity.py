@@ -90,13 +90,14 @@ def _case(kind, seed=3):
         D.derive(mesh, p)
         parts = D.fill_freestream(mesh, p, ppc=100.0, temperature=1.0, seed=seed)
     elif kind == "box_crowded":
-        # ~600 particles per cell: more than a wave's LDS stage holds, so the
-        # one-wave-per-cell kernel works on the cell in HBM
+        # ~400 particles per cell: more than a wave's LDS stage holds, so the
+        # one-wave-per-cell kernel works on the cell in HBM (with its
+        # per-particle conflict words in LDS)
         mesh = D.Mesh.box_tets(2)
         p = D.params_box(0.05)
-        p.Npinitial = 600
+        p.Npinitial = 400
         D.derive(mesh, p)
-        parts = D.fill_freestream(mesh, p, ppc=600.0, temperature=1.0, seed=seed)
+        parts = D.fill_freestream(mesh, p, ppc=400.0, temperature=1.0, seed=seed)
     elif kind == "box_vhs":
         # variable hard spheres (omega 0.74) with many candidates per cell:
         # the acceptance test's FP32 estimate and its exact fallback
