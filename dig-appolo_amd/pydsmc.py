@@ -110,7 +110,7 @@ EXPORTED = [
     "dsmc_macro_get", "dsmc_statistic_pre", "dsmc_accum_get", "dsmc_cell_scalars_get", "dsmc_surface_get",
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
     "dsmc_batch_create", "dsmc_case_info", "dsmc_case_surface_fields", "dsmc_fill_device",
-    "dsmc_run_async", "dsmc_run_query", "dsmc_engine_estimate", "dsmc_server_create", "dsmc_server_destroy",
+    "dsmc_run_async", "dsmc_run_query", "dsmc_run_on_stream", "dsmc_engine_estimate", "dsmc_server_create", "dsmc_server_destroy",
     "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_request",
     "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
@@ -446,6 +446,21 @@ class Engine:
         ms = C.c_double()
         check(lib().dsmc_last_run_ms(self.h, C.byref(ms)), "last_run_ms")
         return ms.value
+
+    def run_async(self, first: int, last: int):
+        """Enqueue steps first..last on the engine's stream and return."""
+        check(lib().dsmc_run_async(self.h, first, last), f"run_async {first}..{last}")
+
+    def run_on_stream(self, first: int, last: int, stream: int):
+        """Enqueue steps first..last ordered on a caller's HIP stream (the
+        hipStream_t handle as an integer)."""
+        check(lib().dsmc_run_on_stream(self.h, first, last, C.c_void_p(stream)), f"run_on_stream {first}..{last}")
+
+    def run_query(self) -> bool:
+        """Whether the last asynchronous run has completed (raises its errors)."""
+        done = C.c_int32()
+        check(lib().dsmc_run_query(self.h, C.byref(done)), "run_query")
+        return bool(done.value)
 
     def track(self, istep: int):
         check(lib().dsmc_track(self.h, istep), "track")

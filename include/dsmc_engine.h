@@ -246,6 +246,13 @@ int dsmc_run(dsmc_engine *e, int32_t first, int32_t last);
  * sets *done once they have completed (then reporting their errors). */
 int dsmc_run_async(dsmc_engine *e, int32_t first, int32_t last);
 int dsmc_run_query(dsmc_engine *e, int32_t *done);
+/* dsmc_run_async ordered on a caller's HIP stream (`stream` is a hipStream_t;
+ * no reference counterpart): the steps start after the work already enqueued
+ * on `stream`, run on the engine's own streams (and step graphs), and work
+ * enqueued on `stream` afterwards starts after them.  Returns without
+ * synchronising; dsmc_run_query reports completion and errors.  A stream
+ * under graph capture is refused: the engine replays its own step graphs. */
+int dsmc_run_on_stream(dsmc_engine *e, int32_t first, int32_t last, void *stream);
 /* Move only (the minimum slice): advection of every particle for one dtime
  * (src/ProcessDSMC.cpp:1946-2017) followed by the regroup. */
 int dsmc_track(dsmc_engine *e, int32_t istep);

@@ -221,6 +221,55 @@ def test_empty_start_fills_from_the_inlet():
         assert st_g[k] == st_c[k], (k, st_g[k], st_c[k])
 
 
+def _hip_runtime():
+    """The HIP runtime the engine library is linked against (from this
+    process's mappings), for creating a caller's stream in a test."""
+    import ctypes
+    D.lib()
+    with open("/proc/self/maps") as f:
+        paths = {ln.split()[-1] for ln in f if "libamdhip64" in ln}
+    assert paths, "HIP runtime not mapped"
+    return ctypes.CDLL(sorted(paths)[0])
+
+
+def test_run_on_a_caller_stream_matches_run():
+    """Steps ordered on a caller's HIP stream (dsmc_run_on_stream, with work
+    enqueued on that stream before and after the run) end in the state
+    dsmc_run reaches, and are complete once the caller's stream is; a stream
+    under graph capture is refused before anything is enqueued on it."""
+    import ctypes
+    mesh, p, parts = _case("capsule")
+    print(f"[run_on_stream] capsule cells={mesh.ncell} particles={parts.n}", flush=True)
+    a = D.Engine(mesh, p, seed=9)
+    a.upload(parts)
+    a.run(1, 6)
+    b = D.Engine(mesh, p, seed=9)
+    b.upload(parts)
+    hip = _hip_runtime()
+    st, buf, g = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0
+    assert hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(1 << 20)) == 0
+    assert hip.hipMemsetAsync(buf, 1, ctypes.c_size_t(1 << 20), st) == 0
+    b.run_on_stream(1, 6, st.value)
+    assert hip.hipMemsetAsync(buf, 2, ctypes.c_size_t(1 << 20), st) == 0
+    assert hip.hipStreamSynchronize(st) == 0
+    assert b.run_query()  # the caller's stream waited for the run
+    _same_particles(a.download(), b.download())
+    for fam in (0, 1, 2):
+        assert np.array_equal(a.accum(fam), b.accum(fam)), fam
+    assert hip.hipStreamBeginCapture(st, 2) == 0  # relaxed capture mode
+    with pytest.raises(D.DsmcError, match="captured"):
+        b.run_on_stream(7, 8, st.value)
+    assert hip.hipStreamEndCapture(st, ctypes.byref(g)) == 0
+    if g.value:
+        hip.hipGraphDestroy(g)
+    hip.hipFree(buf)
+    hip.hipStreamDestroy(st)
+    b.run(7, 8)
+    a.run(7, 8)
+    _same_particles(a.download(), b.download())
+
+
 def test_capacity_overflow_is_an_error():
     """Inflow beyond the engine's capacity ends the step with a capacity
     error: the emission is cut at the capacity, so no kernel writes past it,
