@@ -131,11 +131,20 @@ def cpu_baseline(threads: int, workload: str = "box", ppc: float = 20.0, seconds
     orc.set_particles(parts)
     orc.step(1)  # warm caches
     st0 = orc.stats()["particle_steps"]
+    # each step stage by stage, timed like the reference driver's timers
+    # (src/meshImportTest.cpp:205-236): Preprocess = preprocesseffquad +
+    # statistic_macro, Advection, cache2chain, Collision
+    stages = (("Preprocess", D.STAGE_EMIT), ("Advection", D.STAGE_MOVE), ("cache2chain", D.STAGE_SORT),
+              ("Collision", D.STAGE_COLLIDE), ("Preprocess", D.STAGE_SAMPLE))
+    split = {"Preprocess": 0.0, "Advection": 0.0, "cache2chain": 0.0, "Collision": 0.0}
     t0 = time.perf_counter()
     steps = 0
     while True:
         steps += 1
-        orc.step(1 + steps)
+        for name, mask in stages:
+            ts = time.perf_counter()
+            orc.step(1 + steps, mask)
+            split[name] += time.perf_counter() - ts
         if time.perf_counter() - t0 > seconds_target or steps >= 50:
             break
     dt = time.perf_counter() - t0
@@ -156,6 +165,7 @@ def cpu_baseline(threads: int, workload: str = "box", ppc: float = 20.0, seconds
         "cores": threads,
         "kind": "port",
         "sample": f"{what}, {steps} full steps in {dt:.1f} s, {threads} threads",
+        "stage_ms_per_step": {k: round(1e3 * v / steps, 2) for k, v in split.items()},
     }
 
 
