@@ -352,7 +352,8 @@ def run_serve(args):
     rng = np.random.default_rng(20261203)
     arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(specs)))
     arrivals -= arrivals[0]
-    srv = D.Server(budget_bytes=0, chunk_steps=args.serve_chunk, max_running=args.serve_max_running)
+    srv = D.Server(budget_bytes=0, chunk_steps=args.serve_chunk, max_running=args.serve_max_running,
+                   batching=args.serve_batch)
     t0 = time.perf_counter()
     nxt, ids, last_log = 0, [], t0
     while True:
@@ -386,6 +387,8 @@ def run_serve(args):
                                f"Poisson arrivals {args.serve_rate:g}/s, {args.serve_steps}-step solves, "
                                f"chunks of {args.serve_chunk} steps, HBM-footprint admission"
                                + (f", at most {args.serve_max_running} running" if args.serve_max_running else "")
+                               + (f", up to {args.serve_batch} requests per engine" if args.serve_batch > 1 else
+                                  ", one engine per request")
                                + f", {os.environ.get('GPU_MAX_HW_QUEUES', '4')} hardware queues"
                                + ", mixed precision",
                    "particles_initial": int(sum(i["particles"] for i in infos)), "parallelism": "single GPU"},
@@ -407,6 +410,8 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=1000)
     ap.add_argument("--serve-chunk", type=int, default=50)
     ap.add_argument("--serve-max-running", type=int, default=8, help="requests running at once (0: no limit)")
+    ap.add_argument("--serve-batch", type=int, default=4,
+                    help="requests admitted together share one batch engine, up to this many")
     ap.add_argument("--serve-hw-queues", type=int, default=16,
                     help="HIP hardware queues of the serving process (sets GPU_MAX_HW_QUEUES; 0 keeps the environment's)")
     ap.add_argument("--serve-scale", type=float, default=0.1, help="multiplies every request's cell count")

@@ -82,11 +82,12 @@ class Tuning(C.Structure):
 
 class CaseInfo(C.Structure):
     _fields_ = [("ncase", C.c_int32), ("cell_begin", C.c_int32), ("cell_end", C.c_int32),
-                ("face_begin", C.c_int32), ("face_end", C.c_int32), ("particles", C.c_int64)]
+                ("face_begin", C.c_int32), ("face_end", C.c_int32), ("particles", C.c_int64),
+                ("particle_steps", C.c_int64)]
 
 
 class RequestInfo(C.Structure):
-    _fields_ = [("state", C.c_int32), ("steps", C.c_int32), ("steps_done", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("state", C.c_int32), ("steps", C.c_int32), ("steps_done", C.c_int32), ("kase", C.c_int32),
                 ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_finish", C.c_double),
                 ("estimate", C.c_int64), ("footprint", C.c_int64), ("particles", C.c_int64),
                 ("particle_steps", C.c_int64), ("error", C.c_char * 256)]
@@ -111,7 +112,7 @@ EXPORTED = [
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
     "dsmc_batch_create", "dsmc_case_info", "dsmc_case_surface_fields", "dsmc_fill_device",
     "dsmc_run_async", "dsmc_run_query", "dsmc_run_on_stream", "dsmc_engine_estimate", "dsmc_server_create", "dsmc_server_destroy",
-    "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_request",
+    "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_set_batching", "dsmc_server_request",
     "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
     "dsmc_device_math_check", "dsmc_device_accept_check", "dsmc_ns_apply", "dsmc_write_fluent_cells", "dsmc_write_fluent_walls",
@@ -589,13 +590,15 @@ class Server:
     """Serving loop with continuous batching (see dsmc_server_create)."""
 
     def __init__(self, budget_bytes: int = 0, chunk_steps: int = 50, auto_release: bool = True,
-                 max_running: int = 0):
+                 max_running: int = 0, batching: int = 1):
         self.h = C.c_void_p()
         check(lib().dsmc_server_create(C.c_int64(budget_bytes), int(chunk_steps), int(auto_release),
                                        C.byref(self.h)), "server_create")
         self.meshes = {}
         if max_running:
             check(lib().dsmc_server_set_max_running(self.h, int(max_running)), "server_set_max_running")
+        if batching != 1:
+            check(lib().dsmc_server_set_batching(self.h, int(batching)), "server_set_batching")
 
     def close(self):
         if getattr(self, "h", None):
@@ -622,12 +625,14 @@ class Server:
     def request(self, rid: int) -> dict:
         i = RequestInfo()
         check(lib().dsmc_server_request(self.h, int(rid), C.byref(i)), "server_request")
-        d = {f: getattr(i, f) for f, _ in RequestInfo._fields_ if f not in ("pad", "error")}
+        d = {f: getattr(i, f) for f, _ in RequestInfo._fields_ if f != "error"}
         d["error"] = i.error.decode(errors="replace")
         return d
 
     def engine(self, rid: int, params: Params) -> Engine:
-        """The engine of a finished, unreleased request (borrowed)."""
+        """The engine of a finished, unreleased request (borrowed).  For a
+        batched request it is the batch engine: the request's results are its
+        case (request(rid)["kase"]) of it."""
         h = C.c_void_p()
         check(lib().dsmc_server_engine(self.h, int(rid), C.byref(h)), "server_engine")
         e = _Borrowed.__new__(_Borrowed)
@@ -635,6 +640,11 @@ class Server:
         e.params = params
         nc, nf = self.meshes[rid]
         e.mesh = types.SimpleNamespace(ncell=nc, nface=nf)
+        n = e.ncase()
+        if n > 1:
+            # batch-wide sizes: the last case's range rounded up to 64 cells
+            last = e.case_info(n - 1)
+            e.mesh = types.SimpleNamespace(ncell=(last["cell_end"] + 63) // 64 * 64, nface=last["face_end"])
         return e
 
     def release(self, rid: int):

@@ -352,6 +352,7 @@ typedef struct dsmc_case_info_t {
   int32_t cell_begin, cell_end; /* the case's own cells in the batch */
   int32_t face_begin, face_end; /* the case's faces in the batch */
   int64_t particles;            /* particles of the case now */
+  int64_t particle_steps;       /* sum over steps of the case's count at step start */
 } dsmc_case_info_t;
 int dsmc_case_info(dsmc_engine *e, int32_t kase, dsmc_case_info_t *info);
 /* dsmc_surface_fields for one case of a batch (istep: the case's step). */
@@ -365,10 +366,13 @@ int dsmc_engine_estimate(const dsmc_mesh_view *mesh, const dsmc_params *p, int64
 /* ---- serving (no reference counterpart) ------------------------------------
  * A queue of case requests served with continuous batching: requests are
  * admitted in arrival order while their estimated footprint fits the HBM
- * budget (<= 0: 85% of the free device memory); each admitted request runs
- * in its own engine on its own stream, advanced `chunk_steps` at a time
- * with asynchronous runs, so admitted requests overlap on the GPU; a finished
- * request frees its memory (auto_release) and queued ones are admitted.
+ * budget (<= 0: 85% of the free device memory); admitted requests run in
+ * engines on their own streams, advanced `chunk_steps` at a time with
+ * asynchronous runs, so they overlap on the GPU.  By default each request
+ * gets its own engine; with dsmc_server_set_batching(k) the queued requests
+ * admitted together (up to k, consecutive in arrival order, same step count)
+ * share one batch engine.  A finished request frees its memory (auto_release;
+ * a batch engine goes with its last request) and queued ones are admitted.
  * Results are bit-identical to the same case run alone.  dsmc_server_pump
  * never blocks: call it repeatedly until every request has finished.
  * dsmc_server_submit takes ownership of `mesh`; capacity <= 0 picks
@@ -380,7 +384,8 @@ typedef struct dsmc_server dsmc_server;
 #define DSMC_REQ_FAILED 3
 #define DSMC_REQ_RELEASED 4
 typedef struct dsmc_request_info {
-  int32_t state, steps, steps_done, pad;
+  int32_t state, steps, steps_done;
+  int32_t kase; /* the request's case in its engine (dsmc_case_info) */
   double t_submit, t_admit, t_finish;
   int64_t estimate, footprint, particles, particle_steps;
   char error[256];
@@ -395,8 +400,16 @@ int dsmc_server_pump(dsmc_server *s, int32_t *running, int32_t *queued, int32_t 
  * at the same time share the device's hardware queues, so a cap near their
  * number keeps each engine's kernels from waiting behind the others'. */
 int dsmc_server_set_max_running(dsmc_server *s, int32_t max_running);
+/* Up to max_cases requests per engine (1, the default: one engine each).
+ * Requests queued at the same admission, consecutive in arrival order and
+ * with the same step count, then share one batch engine (dsmc_batch_create):
+ * fewer, fuller launches than one engine per request.  The running cap still
+ * counts requests. */
+int dsmc_server_set_batching(dsmc_server *s, int32_t max_cases);
 int dsmc_server_request(dsmc_server *s, int64_t id, dsmc_request_info *info);
-/* The engine of a finished (not released) request, to read its results. */
+/* The engine of a finished (not released) request, to read its results; a
+ * batched request's results are its case (dsmc_request_info.kase) of that
+ * engine. */
 int dsmc_server_engine(dsmc_server *s, int64_t id, dsmc_engine **engine);
 int dsmc_server_release(dsmc_server *s, int64_t id);
 

@@ -1,6 +1,7 @@
 """Batched sweep and serving on the GPU: the generated cases, the engine
-footprint estimate, and the server's continuous batching (each request's
-result bit-identical alone and in the mix)."""
+footprint estimate, and the server's continuous batching, with an engine per
+request and with requests sharing batch engines (each request's result
+bit-identical alone and in the mix)."""
 import os
 import sys
 
@@ -61,6 +62,45 @@ def test_served_requests_match_alone():
             assert np.array_equal(served.accum(fam), alone.accum(fam)), fam
         assert info["particle_steps"] == alone.stats()["particle_steps"]
         alone.close()
+        srv.release(rid)
+    srv.close()
+
+
+def test_batched_requests_match_alone():
+    """With batching, the requests queued together (same step count) share
+    one batch engine, so their kernels are the same launches, and each ends
+    bit-identical to its case run alone; a request with another step count
+    gets an engine of its own."""
+    steps, seeds = 20, [701, 702, 703, 704]
+    specs = [SW.case_spec(i, 0.02) for i in (7, 8, 9, 10)]
+    nsteps = [steps, steps, steps, 12]
+    srv = D.Server(budget_bytes=0, chunk_steps=6, auto_release=False, batching=4)
+    ids = []
+    for sp, sd, st in zip(specs, seeds, nsteps):
+        m, p = SW.build_case(sp, D.PREC_MIXED, steps=st)
+        ids.append(srv.submit(m, p, seed=sd, ppc=sp["ppc"], steps=st))
+    while srv.pump()[2] < len(ids):
+        pass
+    infos = [srv.request(i) for i in ids]
+    print(f"[serve batched] kase {[i['kase'] for i in infos]} admits {[round(i['t_admit'], 3) for i in infos]}",
+          flush=True)
+    assert [i["kase"] for i in infos] == [0, 1, 2, 0]
+    assert infos[0]["t_admit"] == infos[1]["t_admit"] == infos[2]["t_admit"]
+    for rid, info, sp, sd, st in zip(ids, infos, specs, seeds, nsteps):
+        assert info["state"] == D.REQ_DONE, info["error"]
+        m, p = SW.build_case(sp, D.PREC_MIXED, steps=st)
+        alone = D.Engine(m, p, seed=sd)
+        alone.fill_device(sp["ppc"], sd)
+        alone.run(1, st)
+        served = srv.engine(rid, p)
+        k = info["kase"]
+        ci = served.case_info(k)
+        _same(served.case_particles(served.download(), k), alone.download())
+        for fam in (0, 1, 2):
+            assert np.array_equal(served.accum(fam)[ci["cell_begin"]:ci["cell_end"]], alone.accum(fam)), fam
+        assert info["particle_steps"] == alone.stats()["particle_steps"]
+        alone.close()
+    for rid in ids:
         srv.release(rid)
     srv.close()
 
