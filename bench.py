@@ -513,6 +513,10 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(threads, args.workload, args.ppc)
         log(f"[bench] cpu baseline: {cpu['value']:.4g} particle-steps/s on {threads} threads")
+        if threads > 1:
+            one = cpu_baseline(1, args.workload, args.ppc, seconds_target=6.0)
+            cpu["single_thread"] = {"value": one["value"], "sample": one["sample"]}
+            log(f"[bench] cpu baseline: {one['value']:.4g} particle-steps/s on 1 thread")
 
     out = {
         "metric": METRIC,

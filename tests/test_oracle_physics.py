@@ -448,3 +448,33 @@ def test_ns_apply_refills_an_empty_cell_with_the_ns_maxwellian():
     T = 2.0 / (3.0 * n) * (np.sum(v * v) - n * np.dot(u, u))
     assert np.allclose(u, [0.3, -0.2, 0.1], atol=1e-12) and abs(T - 1.5) < 1e-11
     assert abs(4.0 / p2.dr / n * q.ir[sel].sum() - 0.8) < 1e-11
+
+
+def test_oracle_threads_do_not_change_results():
+    """The oracle's OpenMP paths (advection over cell ranges, the parallel
+    cache2chain, collisions over cells) give the single-thread state bit for
+    bit: recv_cache order and every bucket's order are kept.  The box has
+    enough movers per step (~700k) for the parallel cache2chain."""
+    mesh = D.Mesh.box_tets(16)
+    p = D.params_box(0.05)
+    p.Npinitial = 50
+    D.derive(mesh, p)
+    parts = D.fill_freestream(mesh, p, ppc=50.0, temperature=1.0, seed=5)
+    out = []
+    try:
+        for nt in (1, 4):
+            O.set_threads(nt)
+            o = O.Oracle(mesh, p, seed=5)
+            o.set_particles(parts)
+            for s in (1, 2, 3):
+                o.step(s)
+            out.append((o.particles(), [o.accum(f) for f in (0, 1, 2)], o.stats()))
+    finally:
+        O.set_threads(1)
+    (a, aa, sa), (b, ab, sb) = out
+    for f in ("x", "y", "z", "vx", "vy", "vz", "ir"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    assert np.array_equal(a.cell, b.cell) and np.array_equal(a.id, b.id)
+    for fa, fb in zip(aa, ab):
+        assert np.array_equal(fa, fb)
+    assert sa["crossings"] == sb["crossings"] > 500000
