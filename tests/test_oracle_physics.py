@@ -478,3 +478,33 @@ def test_oracle_threads_do_not_change_results():
     for fa, fb in zip(aa, ab):
         assert np.array_equal(fa, fb)
     assert sa["crossings"] == sb["crossings"] > 500000
+
+
+def test_staged_step_counts_particle_steps_once():
+    """A step run stage by stage (one call per reference driver stage, as the
+    CPU baseline times it) counts its particles once, at the step's start,
+    like a full step; a repeated or earlier stage opens a new step."""
+    m = D.Mesh.capsule(a=4, c=8, layers=4, diameter=3.912)
+    p = D.params_apollo(monatomic=True)
+    p.Npinitial = 6
+    D.derive(m, p)
+    parts = D.fill_freestream(m, p, ppc=6.0, seed=3)
+    full, staged = O.Oracle(m, p, seed=4), O.Oracle(m, p, seed=4)
+    full.set_particles(parts)
+    staged.set_particles(parts)
+    stages = (D.STAGE_EMIT, D.STAGE_MOVE, D.STAGE_SORT, D.STAGE_COLLIDE, D.STAGE_SAMPLE)
+    expect = 0
+    for s in (1, 2, 3):
+        expect += staged.stats()["particles"]
+        full.step(s)
+        for mask in stages:
+            staged.step(s, mask)
+        a, b = staged.stats(), full.stats()
+        assert a["particle_steps"] == b["particle_steps"] == expect, (s, a, b)
+        assert a["emitted"] == b["emitted"] and a["particles"] == b["particles"]
+    # the same stage again is a new step, counted again
+    c1 = staged.stats()["particles"]
+    staged.step(4, D.STAGE_EMIT)
+    c2 = staged.stats()["particles"]
+    staged.step(4, D.STAGE_EMIT)
+    assert staged.stats()["particle_steps"] == expect + c1 + c2
