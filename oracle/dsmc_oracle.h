@@ -29,6 +29,29 @@ typedef struct dsmc_oracle dsmc_oracle;
 int oracle_create(const dsmc_mesh_view *mesh, const dsmc_params *p, uint64_t seed, dsmc_oracle **out);
 void oracle_destroy(dsmc_oracle *o);
 int oracle_set_threads(int32_t n);
+/* Plants one known mistake in every oracle created afterwards (0 removes
+ * it), so that a test can show that its closed-form check fails under that
+ * mistake.  Never set outside those tests.
+ *   1 VHS prefactor with the molecular mass where the reduced mass belongs
+ *     (src/ProcessDSMC.cpp:2626-2630)
+ *   2 VHS collision rate with the hard-sphere speed power (c_r^1 instead of
+ *     c_r^(1-2 eta)) while the prefactor keeps its eta
+ *   3 Larsen-Borgnakke exchange offered to the first partner only
+ *     (src/ProcessDSMC.cpp:2700-2712 offers it to both)
+ *   4 Larsen-Borgnakke energy split with the exponent 1/(1.5 - omega)
+ *     instead of 1/(2.5 - omega)
+ *   5 inflow normal speed drawn without the flux weight v_n
+ *     (src/ProcessDSMC.cpp:2435-2441)
+ *   6 inflow tangential speeds scaled by T instead of sqrt(T)
+ *     (src/ProcessDSMC.cpp:2446-2449) */
+#define ORACLE_PLANT_NONE 0
+#define ORACLE_PLANT_VHS_MASS 1
+#define ORACLE_PLANT_VHS_HS_POWER 2
+#define ORACLE_PLANT_LB_ONE_PARTNER 3
+#define ORACLE_PLANT_LB_EXPONENT 4
+#define ORACLE_PLANT_INFLOW_NO_FLUX_WEIGHT 5
+#define ORACLE_PLANT_INFLOW_TANGENT_SCALE 6
+int oracle_plant(int32_t which);
 int oracle_set_particles(dsmc_oracle *o, const dsmc_particles_soa *soa);
 int64_t oracle_count(dsmc_oracle *o);
 int oracle_get_particles(dsmc_oracle *o, dsmc_particles_soa *soa);

@@ -5,6 +5,7 @@ check and time against; the engine never calls it.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import sys
@@ -34,6 +35,23 @@ def lib():
 
 def set_threads(n: int):
     lib().oracle_set_threads(int(n))
+
+
+PLANT_VHS_MASS, PLANT_VHS_HS_POWER = 1, 2
+PLANT_LB_ONE_PARTNER, PLANT_LB_EXPONENT = 3, 4
+PLANT_INFLOW_NO_FLUX_WEIGHT, PLANT_INFLOW_TANGENT_SCALE = 5, 6
+
+
+@contextlib.contextmanager
+def planted(which: int):
+    """Oracles created inside the block carry one known mistake (see
+    oracle_plant in dsmc_oracle.h), so a test can show its check fails."""
+    if lib().oracle_plant(int(which)) != 0:
+        raise ValueError(f"unknown plant {which}")
+    try:
+        yield
+    finally:
+        lib().oracle_plant(0)
 
 
 def split_tags(mesh: "D.Mesh") -> np.ndarray:
