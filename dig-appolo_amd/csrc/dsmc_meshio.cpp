@@ -92,6 +92,8 @@ int dsmc_mesh_read_cas(const char* path, double scale, dsmc_mesh** out) {
     if (t.size() < 3 || t[0] == "labels") continue;
     const long zone = hex(t[0]), first = hex(t[1]), last = hex(t[2]);
     if (zone < 0 || first < 0 || last < first - 1) return fail("bad section header: " + line);
+    // a zone's records are numbered from 1 (record k goes to index k - 1)
+    if (zone != 0 && (first < 1 || last < first)) return fail("bad section range: " + line);
     if (zone == 0) {  // declaration of the totals
       if (nodes) {
         nnode = last;

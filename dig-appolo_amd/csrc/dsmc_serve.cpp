@@ -358,6 +358,14 @@ int dsmc_server_release(dsmc_server* s, int64_t id) {
   Request* r = find(s, id);
   if (!r) return set_error(DSMC_ERR_ARG, "dsmc_server_release: no such request");
   if (r->state == REQ_RUNNING) return set_error(DSMC_ERR_STATE, "dsmc_server_release: request still running");
+  if (r->state == REQ_QUEUED) {
+    // a cancel: the request leaves the queue before admission, so no group
+    // (and no batch-mate) ever sees its mesh
+    s->queue.erase(std::remove(s->queue.begin(), s->queue.end(), (size_t)id), s->queue.end());
+    r->error = "cancelled before admission";
+    r->state = REQ_FAILED;
+    r->t_finish = s->now();
+  }
   release(s, *r);
   if (r->state == REQ_DONE) r->state = REQ_RELEASED;
   return DSMC_OK;

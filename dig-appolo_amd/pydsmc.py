@@ -141,14 +141,21 @@ def lib():
     return _lib
 
 
+ERR_ARG, ERR_HIP, ERR_MESH, ERR_CAPACITY, ERR_STATE = -1, -2, -3, -4, -5
+
+
 class DsmcError(RuntimeError):
-    pass
+    """A failed engine call; `code` is its DSMC_ERR_* status."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 def check(rc: int, what: str = ""):
     if rc != DSMC_OK:
         msg = lib().dsmc_last_error().decode(errors="replace")
-        raise DsmcError(f"{what}: error {rc}: {msg}")
+        raise DsmcError(f"{what}: error {rc}: {msg}", rc)
 
 
 def _ptr(a: np.ndarray, ct):

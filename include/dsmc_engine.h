@@ -411,6 +411,11 @@ int dsmc_server_request(dsmc_server *s, int64_t id, dsmc_request_info *info);
  * batched request's results are its case (dsmc_request_info.kase) of that
  * engine. */
 int dsmc_server_engine(dsmc_server *s, int64_t id, dsmc_engine **engine);
+/* Frees a finished request's engine share (state RELEASED).  A request still
+ * queued is cancelled: it leaves the queue, its mesh is freed and its state
+ * becomes FAILED with the error "cancelled before admission"; requests
+ * admitted later are not affected.  A running request is refused
+ * (DSMC_ERR_STATE). */
 int dsmc_server_release(dsmc_server *s, int64_t id);
 
 /* ---- Fluent-format output files -----------------------------------------------

@@ -566,3 +566,36 @@ def test_ns_apply_mixed_counts_match_fp64():
         assert np.array_equal(getattr(a, f), getattr(b, f)), f
     for f in ("vx", "vy", "vz"):
         assert np.allclose(getattr(a, f), getattr(b, f), rtol=1e-5, atol=1e-5), f
+
+
+def test_ns_apply_refuses_oversized_and_infinite_states():
+    """A coupled density that would make the particle total exceed the
+    capacity (every cell near it, so a 32-bit sum of the per-cell counts
+    would wrap) is a capacity error, and an infinite NS field before the NaN
+    stop an argument error; in both cases the particles stay as they were.
+    A NaN density still stops the update at its cell, as in the reference."""
+    mesh, p, parts = _case("box_tets")
+    eng = D.Engine(mesh, p, seed=21)
+    eng.upload(parts)
+    before = eng.download()
+    counts = eng.cell_scalars()["count"].astype(np.float64)
+    ns, _, _ = _ns_fields(mesh, p, counts)
+    ns[:, 0] = np.abs(np.nan_to_num(ns[:, 0], nan=1.0))
+    huge = ns.copy()
+    huge[:, 0] = 1e30
+    with pytest.raises(D.DsmcError) as ei:
+        eng.ns_apply(1, huge)
+    print(f"[ns_apply guard] huge density: {ei.value}", flush=True)
+    assert ei.value.code == D.ERR_CAPACITY
+    inf = ns.copy()
+    inf[7, 4] = np.inf
+    with pytest.raises(D.DsmcError) as ei:
+        eng.ns_apply(1, inf)
+    assert ei.value.code == D.ERR_ARG and "cell 7" in str(ei.value)
+    after = eng.download()
+    _same_particles(after, before)
+    # the same infinite field after a NaN stop is never read
+    late = ns.copy()
+    late[3, 0] = np.nan
+    late[7, 4] = np.inf
+    assert eng.ns_apply(1, late) == 3

@@ -127,3 +127,29 @@ def test_running_cap_holds_requests_in_the_queue():
     for a, b in zip(infos, infos[1:]):
         assert b["t_admit"] >= a["t_finish"]
     srv.close()
+
+
+def test_cancelled_request_leaves_its_batch_mates_running():
+    """A queued request released before admission is cancelled; with
+    batching on, the requests admitted around it share one engine and
+    finish."""
+    steps = 8
+    specs = [SW.case_spec(i, 0.02) for i in (11, 12, 13, 14)]
+    srv = D.Server(budget_bytes=0, chunk_steps=4, auto_release=False, batching=4)
+    ids = []
+    for k, sp in enumerate(specs):
+        m, p = SW.build_case(sp, D.PREC_MIXED, steps=steps)
+        ids.append(srv.submit(m, p, seed=900 + k, ppc=sp["ppc"], steps=steps))
+    srv.release(ids[1])
+    while srv.pump()[2] < len(ids):
+        pass
+    infos = [srv.request(i) for i in ids]
+    print(f"[serve cancel] states {[i['state'] for i in infos]} kase {[i['kase'] for i in infos]}", flush=True)
+    assert infos[1]["state"] == D.REQ_FAILED and "cancelled" in infos[1]["error"]
+    for k in (0, 2, 3):
+        assert infos[k]["state"] == D.REQ_DONE, infos[k]["error"]
+        assert infos[k]["steps_done"] == steps and infos[k]["particle_steps"] > 0
+    assert [infos[k]["kase"] for k in (0, 2, 3)] == [0, 1, 2]
+    for k in (0, 2, 3):
+        srv.release(ids[k])
+    srv.close()

@@ -60,3 +60,17 @@ def test_read_rejects_a_truncated_file(tmp_path):
     path.write_text(src[: src.index("4 8 9 c b 2 0")])
     with pytest.raises(RuntimeError):
         D.Mesh.read_cas(str(path))
+
+
+@pytest.mark.parametrize("old,new", [("(10 (1 1 c 1 3)", "(10 (1 0 c 1 3)"),
+                                      ("(13 (3 1 1 2 4)", "(13 (3 0 1 2 4)")])
+def test_zero_based_section_is_refused(tmp_path, old, new):
+    """A node or face section numbered from 0 would put its first record
+    before the start of the arrays: the reader refuses it."""
+    text = open(os.path.join(GOLDEN, "two_hexes.cas")).read()
+    assert old in text, old
+    bad = tmp_path / "zero_based.cas"
+    bad.write_text(text.replace(old, new, 1))
+    with pytest.raises(D.DsmcError) as ei:
+        D.Mesh.read_cas(str(bad))
+    assert ei.value.code == D.ERR_MESH and "range" in str(ei.value)
