@@ -115,7 +115,7 @@ EXPORTED = [
     "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_set_batching", "dsmc_server_request",
     "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
-    "dsmc_device_math_check", "dsmc_device_accept_check", "dsmc_ns_apply", "dsmc_write_fluent_cells", "dsmc_write_fluent_walls",
+    "dsmc_device_math_check", "dsmc_device_accept_check", "dsmc_ns_apply", "dsmc_ns_apply_noalloc", "dsmc_write_fluent_cells", "dsmc_write_fluent_walls",
     "dsmc_mesh_read_cas", "dsmc_mesh_write_cas",
     "dsmc_copy_probe", "dsmc_last_error",
 ]
@@ -565,6 +565,16 @@ class Engine:
         check(lib().dsmc_ns_apply(self.h, int(istep), a.ctypes.data_as(C.POINTER(C.c_double)),
                                   None if acc is None else acc.ctypes.data_as(C.POINTER(C.c_uint8)),
                                   C.byref(nan_cell)), "ns_apply")
+        return nan_cell.value
+
+    def ns_apply_noalloc(self, istep: int, ns_device: int, accepted_device: int | None = None) -> int:
+        """ns_apply on NS fields already in device memory (addresses of
+        ncell x 6 doubles and ncell bytes), with no device allocation
+        (dsmc_ns_apply_noalloc)."""
+        nan_cell = C.c_int32()
+        check(lib().dsmc_ns_apply_noalloc(self.h, int(istep), C.c_void_p(ns_device),
+                                          C.c_void_p(accepted_device) if accepted_device else None,
+                                          C.byref(nan_cell)), "ns_apply_noalloc")
         return nan_cell.value
 
     def autotune(self, first_step: int, steps_per_trial: int = 2) -> tuple[dict, int]:

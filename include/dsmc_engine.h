@@ -454,6 +454,19 @@ int dsmc_write_fluent_walls(const char *path, const dsmc_mesh_view *mesh, const 
  * *nan_cell receives that cell or -1.  istep (> 0) keys the random streams.
  * New particles take the next ids of their case.  Call between steps. */
 int dsmc_ns_apply(dsmc_engine *e, int32_t istep, const double *ns, const uint8_t *accepted, int32_t *nan_cell);
+/* dsmc_ns_apply that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  ns_device (ncell x 6 doubles)
+ * and accepted_device (ncell bytes, or NULL) are device memory of the
+ * engine's device; results are those of dsmc_ns_apply on the same inputs,
+ * bit for bit.  Its scratch (the call's counters and the replication
+ * offsets) is a fixed 16 MiB array per device, held until the process ends,
+ * so engines of at most 4,194,304 cells are accepted (DSMC_ERR_CAPACITY
+ * beyond).  Every call on one device shares that array: calls that overlap
+ * on one device race (finish one, by its return, before starting another).
+ * The call synchronises the engine's stream, so it cannot be graph-captured. */
+int dsmc_ns_apply_noalloc(dsmc_engine *e, int32_t istep, const double *ns_device, const uint8_t *accepted_device,
+                          int32_t *nan_cell);
 
 /* Bytes of device memory the engine holds. */
 int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
