@@ -71,7 +71,13 @@ def make_apollo(ppc: float, mixed: bool):
     return mesh, p
 
 
-def run_engine(mesh, p, parts, steps, warmup, seed=20261203, autotune=True, tuning=None):
+def run_engine(mesh, p, parts, steps, warmup, seed=20261203, autotune=True, tuning=None, profile_pass=True):
+    """Times `steps` full steps after `warmup` untimed ones, the way the
+    engine runs in production: no profiling, steps replayed as HIP graphs
+    (both buffer parities captured during set-up).  Then, unless
+    profile_pass is off, a separate profiled pass of `steps` more steps
+    (eager launches with a HIP event after each kernel group) gives the
+    per-kernel split; it is labelled as such and never enters `value`."""
     eng = D.Engine(mesh, p, seed=seed)
     eng.upload(parts)
     tune_steps = 0
@@ -85,18 +91,28 @@ def run_engine(mesh, p, parts, steps, warmup, seed=20261203, autotune=True, tuni
         # set-up: the autotuner times live steps (results do not depend on the
         # tuning it picks), then the case restarts from the same particles
         tuning, tune_steps = eng.autotune(1, 2)
-        eng.upload(parts)
         log(f"[bench] autotune ({tune_steps} steps): {tuning}")
+    # set-up: capture the step-pair graphs of both buffer parities (each run
+    # starts with one eager step, which flips the parity), then restart from
+    # the same particles; the graphs stay valid across an upload
+    eng.upload(parts)
+    eng.run(1, 3)
+    eng.run(4, 6)
+    eng.upload(parts)
     if warmup > 0:
         eng.run(1, warmup)
     st0 = eng.stats()
-    eng.profile(True)
     t0 = time.perf_counter()
     ms_events = eng.run(warmup + 1, warmup + steps)
     wall = time.perf_counter() - t0
     st1 = eng.stats()
-    kt = eng.kernel_times()
     psteps = st1["particle_steps"] - st0["particle_steps"]
+    kt, prof_ms = None, None
+    if profile_pass:
+        eng.profile(True)
+        prof_ms = eng.run(warmup + steps + 1, warmup + 2 * steps)
+        kt = eng.kernel_times()
+        eng.profile(False)
     res = {
         "particle_steps": psteps,
         "event_ms": ms_events,
@@ -104,6 +120,7 @@ def run_engine(mesh, p, parts, steps, warmup, seed=20261203, autotune=True, tuni
         "value": psteps / (ms_events / 1e3),
         "ms_per_step": ms_events / steps,
         "kernels": kt,
+        "profiled_ms_per_step": None if prof_ms is None else prof_ms / steps,
         "stats": st1,
         "footprint_GB": eng.footprint() / 1e9,
         "tuning": tuning,
@@ -113,52 +130,72 @@ def run_engine(mesh, p, parts, steps, warmup, seed=20261203, autotune=True, tuni
     return res
 
 
-def cpu_baseline(threads: int, workload: str = "box", ppc: float = 20.0, seconds_target: float = 15.0):
+def cpu_share():
+    """Host threads for the CPU baseline: every core this process may run
+    on (its affinity mask), capped by OMP_NUM_THREADS when that is set (the
+    share of a shared host given to this GPU), with the numbers behind it."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(aff, int(omp)) if omp.isdigit() and int(omp) > 0 else aff
+    model = "unknown CPU"
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    if model == "unknown CPU":
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return threads, {"affinity_cores": aff, "nproc": os.cpu_count(), "OMP_NUM_THREADS": omp or None,
+                     "model": model}
+
+
+def cpu_baseline(threads: int, mesh, p, parts, what: str, first_step: int = 1, seconds_target: float = 15.0,
+                 max_steps: int = 50, warm: bool = True):
+    """The C++ oracle (a strict restatement of the reference's per-step
+    path) from the state `parts`, on `threads` host threads: full steps,
+    each stage by stage and timed like the reference driver's timers
+    (src/meshImportTest.cpp:205-236: Preprocess = preprocesseffquad +
+    statistic_macro, Advection, cache2chain, Collision), until
+    seconds_target or max_steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
 
-    if workload == "apollo":
-        mesh, p = make_apollo(ppc, False)
-        parts = D.fill_freestream(mesh, p, ppc=ppc, seed=7)
-        what = f"C++ oracle, Apollo surrogate ({mesh.ncell} cells), Npinitial {ppc:g} ({parts.n} particles)"
-    else:
-        n = 24
-        mesh, p = make_case(n, 50.0, False)
-        parts = D.fill_freestream(mesh, p, ppc=50.0, temperature=1.0, seed=7)
-        what = f"C++ oracle, {n}^3x6-tet box ({mesh.ncell} cells), 50 ppc ({parts.n} particles)"
     O.set_threads(threads)
     orc = O.Oracle(mesh, p, seed=7)
     orc.set_particles(parts)
-    orc.step(1)  # warm caches
+    s = first_step
+    if warm:
+        orc.step(s)  # warm caches
+        s += 1
     st0 = orc.stats()["particle_steps"]
-    # each step stage by stage, timed like the reference driver's timers
-    # (src/meshImportTest.cpp:205-236): Preprocess = preprocesseffquad +
-    # statistic_macro, Advection, cache2chain, Collision
     stages = (("Preprocess", D.STAGE_EMIT), ("Advection", D.STAGE_MOVE), ("cache2chain", D.STAGE_SORT),
               ("Collision", D.STAGE_COLLIDE), ("Preprocess", D.STAGE_SAMPLE))
     split = {"Preprocess": 0.0, "Advection": 0.0, "cache2chain": 0.0, "Collision": 0.0}
     t0 = time.perf_counter()
     steps = 0
     while True:
-        steps += 1
         for name, mask in stages:
             ts = time.perf_counter()
-            orc.step(1 + steps, mask)
+            orc.step(s, mask)
             split[name] += time.perf_counter() - ts
-        if time.perf_counter() - t0 > seconds_target or steps >= 50:
+        steps += 1
+        s += 1
+        if time.perf_counter() - t0 > seconds_target or steps >= max_steps:
             break
     dt = time.perf_counter() - t0
     ps = orc.stats()["particle_steps"] - st0
-    model = "unknown CPU"
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    what += f" on {model}"
+    del orc
     return {
         "value": ps / dt,
         "unit": "particle-steps/s",
@@ -167,6 +204,76 @@ def cpu_baseline(threads: int, workload: str = "box", ppc: float = 20.0, seconds
         "sample": f"{what}, {steps} full steps in {dt:.1f} s, {threads} threads",
         "stage_ms_per_step": {k: round(1e3 * v / steps, 2) for k, v in split.items()},
     }
+
+
+def cpu_pair(mesh, p, parts, what, first_step=1, seconds=15.0, single_seconds=6.0, threads=0):
+    """All-share and one-thread CPU baselines of the same state (`threads`
+    > 0 overrides the share)."""
+    share, host = cpu_share()
+    threads = threads or share
+    what = f"{what} on {host['model']}"
+    cpu = cpu_baseline(threads, mesh, p, parts, what, first_step, seconds_target=seconds)
+    cpu["host"] = host
+    log(f"[bench] cpu baseline: {cpu['value']:.4g} particle-steps/s on {threads} threads ({cpu['sample']})")
+    if threads > 1:
+        one = cpu_baseline(1, mesh, p, parts, what, first_step, seconds_target=single_seconds,
+                           warm=False)
+        cpu["single_thread"] = {"value": one["value"], "sample": one["sample"]}
+        log(f"[bench] cpu baseline: {one['value']:.4g} particle-steps/s on 1 thread")
+    return cpu
+
+
+def apollo_record(steps: int = 500, nss: int = 250, ppc: float = 20.0, seed: int = 20261203, cpu: bool = True):
+    """The Apollo case as the reference runs it (monatomic VHS, Ma 5,
+    Kn 0.01, AoA 30, diffuse wall at 300 K, inflow, Npinitial 20 on the
+    372,500-cell surrogate O-grid, `steps` steps with sampling from `nss`):
+    FP64 and mixed, each timed over the whole job and over the developed
+    flow (steps nss+1..steps); the CPU oracle is timed from the FP64
+    engine's state at step nss, on the same host."""
+    t_set = time.perf_counter()
+    mesh, p = make_apollo(ppc, False)
+    p.NSS = nss
+    parts = D.fill_freestream(mesh, p, ppc=ppc, seed=seed)
+    log(f"[bench] apollo: {mesh.ncell} cells, {parts.n} particles (set-up {time.perf_counter() - t_set:.1f} s)")
+    out = {"workload": f"Apollo capsule surrogate hex O-grid ({mesh.ncell} cells), monatomic VHS, Ma 5, Kn 0.01, "
+                       f"AoA 30, diffuse wall 300 K, inflow, Npinitial {ppc:g} ({parts.n} particles at start), "
+                       f"{steps} steps, sampling from step {nss}",
+           "cells": mesh.ncell, "particles_start": parts.n, "steps": steps}
+    developed = None
+    for tag, prec in (("fp64", D.PREC_FP64), ("mixed", D.PREC_MIXED)):
+        q = p.copy()
+        q.precision = prec
+        eng = D.Engine(mesh, q, seed=seed)
+        eng.upload(parts)
+        tuning, _ = eng.autotune(1, 2)
+        eng.upload(parts)
+        eng.run(1, 3)
+        eng.run(4, 6)
+        eng.upload(parts)
+        st0 = eng.stats()["particle_steps"]
+        ms1 = eng.run(1, nss)
+        st1 = eng.stats()["particle_steps"]
+        if tag == "fp64" and cpu:
+            developed = eng.download()
+        ms2 = eng.run(nss + 1, steps)
+        st = eng.stats()
+        rec = {"value_job": round((st["particle_steps"] - st0) / ((ms1 + ms2) / 1e3), 1),
+               "value_developed": round((st["particle_steps"] - st1) / (ms2 / 1e3), 1),
+               "ms_per_step_developed": round(ms2 / (steps - nss), 4),
+               "particles_end": int(st["particles"]), "dropped": int(st["dropped"]), "tuning": tuning}
+        out[tag] = rec
+        log(f"[bench] apollo {tag}: job {rec['value_job']:.4g}, developed flow {rec['value_developed']:.4g} "
+            f"particle-steps/s ({rec['ms_per_step_developed']:.3f} ms/step)")
+        eng.close()
+    out["mixed_speedup_developed"] = round(out["mixed"]["value_developed"] / out["fp64"]["value_developed"], 3)
+    if cpu:
+        c = cpu_pair(mesh, p, developed,
+                     f"C++ oracle, Apollo surrogate from the FP64 engine's state at step {nss} "
+                     f"({developed.n} particles)", first_step=nss + 1)
+        out["cpu_baseline"] = c
+        out["gpu_vs_cpu_developed"] = round(out["fp64"]["value_developed"] / c["value"], 1)
+        out["target"] = "FP64 engine >= 20x the host CPU on the Apollo case"
+    return out
 
 
 def _energy(q):
@@ -422,6 +529,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-apollo", action="store_true", help="skip the Apollo sub-record of the box workload")
     ap.add_argument("--tuning", type=json.loads, default=None,
                     help='launch settings instead of the autotuner, e.g. \'{"move_chunk": 4096, "wide_min": 64}\'')
     ap.add_argument("--validate", choices=["box", "apollo"],
@@ -482,7 +590,8 @@ def main():
         log(f"[bench] mixed: {mixed['value']:.4g} particle-steps/s, {mixed['ms_per_step']:.2f} ms/step")
 
     # roofline of the dominant kernel group, from the HIP events around each
-    # launch over the timed steps
+    # launch over the profiled pass (the same number of steps, right after
+    # the timed ones; eager launches with an event after every kernel group)
     kt = fp64["kernels"]
     groups = {"move": kt["move"][0], "sort": sum(kt[g][0] for g in ("sort_count", "sort_scan", "sort_place",
                                                                    "sort_gather")),
@@ -505,18 +614,22 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "bytes_per_particle": per_particle, "launch_ms": round(launch_ms, 4),
-                "alg_bytes_per_launch": round(per_particle * nmean)}
+                "alg_bytes_per_launch": round(per_particle * nmean),
+                "timing": f"HIP events on the engine's stream around each launch, over a profiled pass of "
+                          f"{args.steps} steps right after the timed ones"}
     step_bytes = sum(alg_bytes(k, 64, 32, 80.0 * ntri, nmean_ppc) for k in ("move", "sort", "collide_sample"))
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(threads, args.workload, args.ppc)
-        log(f"[bench] cpu baseline: {cpu['value']:.4g} particle-steps/s on {threads} threads")
-        if threads > 1:
-            one = cpu_baseline(1, args.workload, args.ppc, seconds_target=6.0)
-            cpu["single_thread"] = {"value": one["value"], "sample": one["sample"]}
-            log(f"[bench] cpu baseline: {one['value']:.4g} particle-steps/s on 1 thread")
+        if args.workload == "apollo":
+            what = f"C++ oracle, Apollo surrogate ({mesh.ncell} cells), Npinitial {args.ppc:g} ({parts.n} particles)"
+        else:
+            what = (f"C++ oracle, the same {args.n}^3x6-tet box ({mesh.ncell} cells), {args.ppc:g} ppc "
+                    f"({parts.n} particles)")
+        cpu = cpu_pair(mesh, p, parts, what, seconds=12.0, threads=args.cpu_threads)
+    apollo = None
+    if args.workload == "box" and not args.no_apollo:
+        apollo = apollo_record(cpu=not args.no_cpu)
 
     out = {
         "metric": METRIC,
@@ -541,6 +654,8 @@ def main():
         "cpu_baseline": cpu,
         "step_alg_GBs": round(step_bytes * nmean / (fp64["ms_per_step"] / 1e3) / 1e9, 1),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]},
+        "kernel_ms_note": "profiled pass (eager launches, an event after each kernel group), not the timed steps",
+        "profiled_ms_per_step": round(fp64["profiled_ms_per_step"], 4),
         "mixed": None if mixed is None else {"value": round(mixed["value"], 1), "dtype": "f32 velocities / f64 positions",
                                              "ms_per_step": round(mixed["ms_per_step"], 4),
                                              "speedup_vs_fp64": round(mixed["value"] / fp64["value"], 3)},
@@ -548,6 +663,8 @@ def main():
         "footprint_GB": round(fp64["footprint_GB"], 2),
         "tuning": {"fp64": fp64["tuning"], "mixed": None if mixed is None else mixed["tuning"]},
         "copy_probe_GBs": None if copy_gbs is None else round(copy_gbs, 1),
+        "timed_path": "graph replay of step pairs, no profiling",
+        "apollo": apollo,
     }
     print(json.dumps(out), flush=True)
 
