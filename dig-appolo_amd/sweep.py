@@ -1,8 +1,10 @@
 """Seeded capsule cases for batched sweeps and the serving queue.
 
 A case is a spherical-cap heat shield + 33 degree cone after-body (diameter
-1 m) in a tet O-grid of 100k-1M cells (hex O-grid layers split 12 tets per
-hex, interior nodes jittered), monatomic VHS gas, Mach uniform in [3, 10],
+1 m) in a tet O-grid of 100k-1M cells (20 hex O-grid layers split 12 tets per
+hex, interior nodes jittered; the surface resolution sets the cell count, so
+the near-wall cells stay about as thick as they are wide and no tet of the
+split is inverted), monatomic VHS gas, Mach uniform in [3, 10],
 Kn log-uniform in [0.005, 0.1], angle of attack uniform in [0, 40] degrees,
 20-100 particles per cell, diffuse wall at T_w, free-stream inflow, mixed
 precision.  Case i is drawn from seed i, so a sweep or a request stream is
@@ -21,10 +23,19 @@ def case_spec(seed: int, scale: float = 1.0, Tw: float | None = None) -> dict:
     """Parameters of case `seed`; `scale` multiplies the cell count."""
     r = np.random.default_rng(seed)
     cells = r.uniform(1.0e5, 1.0e6) * scale
-    a = int(r.integers(5, 12))
+    # 12 tets per hex, `layers` hex layers over a closed surface of box
+    # topology a x a x c (2 a^2 + 4 a c quads, c = 2.48 a); with 20 layers
+    # of growth ratio 1.08 the first layer is about as thick as a surface
+    # quad is wide
+    layers = 20
+    a = max(2, int(round(math.sqrt(cells / (12 * layers * (2 + 4 * 2.48))))))
     c = int(round(2.48 * a))
-    quads = 2 * a * a + 4 * a * c  # closed surface of box topology a x a x c
-    layers = max(4, int(round(cells / (12 * quads))))
+    quads = 2 * a * a + 4 * a * c
+    if a < 5:
+        # small (scaled-down) cases: coarser surfaces keep fewer layers
+        a, c = 5, 12
+        quads = 2 * a * a + 4 * a * c
+        layers = max(2, int(round(cells / (12 * quads))))
     return {
         "seed": int(seed),
         "a": a,

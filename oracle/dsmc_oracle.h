@@ -66,6 +66,11 @@ int oracle_accum_get(dsmc_oracle *o, int32_t family, double *out);
 int oracle_cell_scalars(dsmc_oracle *o, int32_t *count, double *crmax, double *rem_coll, double *rem_emit,
                         uint8_t *sparse_state, int32_t *sparse_steps);
 int oracle_surface_get(dsmc_oracle *o, double *out);
+/* outBoundaryStressHeat (src/ProcessDSMC.cpp:3494-3557): nface x 12 wall
+ * fields in dsmc_surface_fields' layout (stress(3), q_trans, q_rot,
+ * normal(3), pressure, shear(3); zero off the wall), from `tally` (nface x 6)
+ * or, when it is null, the oracle's own steady tallies. */
+int oracle_surface_fields(dsmc_oracle *o, int32_t istep, const double *tally, double *fields);
 int oracle_stats(dsmc_oracle *o, dsmc_stats *s);
 /* Stage timings of the last oracle_step calls (seconds, accumulated):
  * preprocess (emission + sampling), advection, cache2chain, collision —

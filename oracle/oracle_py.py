@@ -172,6 +172,14 @@ class Oracle:
         lib().oracle_surface_get(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
 
+    def surface_fields(self, istep: int, tally: np.ndarray | None = None) -> np.ndarray:
+        out = np.zeros((self.mesh.nface, 12))
+        t = None if tally is None else np.ascontiguousarray(tally, dtype=np.float64).reshape(-1, 6)
+        if lib().oracle_surface_fields(self.h, int(istep), None if t is None else t.ctypes.data_as(C.POINTER(C.c_double)),
+                                       out.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+            raise RuntimeError("oracle_surface_fields failed")
+        return out
+
     def stats(self) -> dict:
         s = D.Stats()
         lib().oracle_stats(self.h, C.byref(s))

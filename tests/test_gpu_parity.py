@@ -408,7 +408,8 @@ def test_autotune_steps_are_ordinary_steps():
 def test_batch_cases_match_alone():
     """Three different cases in one batch engine: every case's particles,
     accumulators and fields are bit-identical to the same case run alone
-    with the same seed; its wall tallies agree to 1e-12."""
+    with the same seed and to the oracle run on that case; its wall tallies
+    agree to 1e-12."""
     kinds = ["capsule", "capsule_tets", "box_tets"]
     seeds = [31, 32, 33]
     cases = [_case(k, seed=4 + i) for i, k in enumerate(kinds)]
@@ -421,6 +422,14 @@ def test_batch_cases_match_alone():
         alone.append({"parts": eng.download(), "acc": [eng.accum(f) for f in (0, 1, 2)],
                       "macro": eng.macro(D.MACRO_OUTPUT), "stats": eng.stats(), "surf": eng.surface()})
         eng.close()
+        orc = O.Oracle(mesh, p, seed=sd)
+        orc.set_particles(parts)
+        for s in range(1, steps + 1):
+            orc.step(s)
+        _same_particles(alone[-1]["parts"], orc.particles())
+        for f in (0, 1, 2):
+            assert np.array_equal(alone[-1]["acc"][f], orc.accum(f)), f
+        assert np.array_equal(alone[-1]["macro"], orc.macro(D.MACRO_OUTPUT), equal_nan=True)
     bat = D.Engine.batch([c[0] for c in cases], [c[1] for c in cases], seeds)
     print(f"[batch] cases={len(cases)} cells={bat.mesh.ncell}", flush=True)
     bat.upload(bat.batch_particles([c[2] for c in cases]))

@@ -8,6 +8,7 @@ import sys
 import numpy as np
 import pytest
 
+import oracle_py as O
 import pydsmc as D
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dig-appolo_amd"))
@@ -152,4 +153,41 @@ def test_cancelled_request_leaves_its_batch_mates_running():
     assert [infos[k]["kase"] for k in (0, 2, 3)] == [0, 1, 2]
     for k in (0, 2, 3):
         srv.release(ids[k])
+    srv.close()
+
+
+def test_fp64_served_requests_match_the_oracle():
+    """FP64 requests served concurrently (the first two sharing a batch
+    engine, the third in its own) end bit-identical to the oracle run on
+    each case: the server's device fill, batching and chunked asynchronous
+    runs change nothing of the reference path."""
+    steps, seeds = 8, [811, 812, 813]
+    specs = [SW.case_spec(i, 0.01) for i in (21, 22, 23)]
+    srv = D.Server(budget_bytes=0, chunk_steps=3, auto_release=False, batching=2)
+    ids = []
+    for sp, sd in zip(specs, seeds):
+        m, p = SW.build_case(sp, D.PREC_FP64, steps=steps)
+        ids.append(srv.submit(m, p, seed=sd, ppc=sp["ppc"], steps=steps))
+    while srv.pump()[2] < len(ids):
+        pass
+    infos = [srv.request(i) for i in ids]
+    print(f"[serve fp64 vs oracle] kase {[i['kase'] for i in infos]} particles {[i['particles'] for i in infos]}",
+          flush=True)
+    assert [i["kase"] for i in infos] == [0, 1, 0]
+    for rid, info, sp, sd in zip(ids, infos, specs, seeds):
+        assert info["state"] == D.REQ_DONE, info["error"]
+        m, p = SW.build_case(sp, D.PREC_FP64, steps=steps)
+        orc = O.Oracle(m, p, seed=sd)
+        orc.set_particles(D.fill_freestream(m, p, ppc=sp["ppc"], seed=sd))
+        for s in range(1, steps + 1):
+            orc.step(s)
+        served = srv.engine(rid, p)
+        k = info["kase"]
+        ci = served.case_info(k)
+        got = served.case_particles(served.download(), k)
+        _same(got, orc.particles())
+        assert np.array_equal(served.accum(0)[ci["cell_begin"]:ci["cell_end"]], orc.accum(0))
+        assert info["particle_steps"] == orc.stats()["particle_steps"]
+    for rid in ids:
+        srv.release(rid)
     srv.close()

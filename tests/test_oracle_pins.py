@@ -18,6 +18,8 @@ parameters, and the same check must then reject it.
 * Normalisation (setMeshMessage / setParticleMessage,
   src/meshImport.cpp:1315-1402): n_ref from the hard-sphere mean free path,
   Neff, dtime and the VHS prefactor.
+* Wall fields (outBoundaryStressHeat, :3494-3557): the wall pressure of an
+  equilibrium gas is n k T, and specular walls pass no heat.
 """
 import math
 
@@ -315,3 +317,59 @@ def test_normalisation_check_catches_mistakes(field, mistake):
     err = _normalisation_errors(q, n)
     key = {"coll_prefactor": "prefactor"}.get(field, field)
     assert abs(err[key]) > 1e-3, (field, err)
+
+
+# ---------------------------------------------------------------------------
+# Wall fields (outBoundaryStressHeat)
+# ---------------------------------------------------------------------------
+def _box_wall_run(model, T, seed, steps=60):
+    """Equilibrium box with steady wall tallies from step 1; returns the
+    oracle, its parameters, the gas's density over n_ref and its sample
+    temperature (units of T_ref)."""
+    m = D.Mesh.box_tets(4)
+    p = D.params_box(0.05)
+    p.Npinitial = 40
+    p.wall_model = model
+    p.T_wall = 273.15 * T
+    D.derive(m, p)
+    p.surface_stats, p.NSS = 1, 0
+    parts = D.fill_freestream(m, p, ppc=40, temperature=T, seed=seed)
+    c = [parts.vx - parts.vx.mean(), parts.vy - parts.vy.mean(), parts.vz - parts.vz.mean()]
+    t_sample = 2.0 / 3.0 * float(np.mean(c[0] ** 2 + c[1] ** 2 + c[2] ** 2))
+    o = O.Oracle(m, p, seed=5)
+    o.set_particles(parts)
+    for s in range(1, steps + 1):
+        o.step(s)
+    return m, p, o, parts.n / (p.Npinitial * m.ncell), t_sample
+
+
+def _wall_means(m, f):
+    w = np.linalg.norm(f[:, 5:8], axis=1) > 0.5
+    area = m.arrays()["face_area"][w]
+    return (float(np.sum(f[w, 8] * area) / area.sum()), float(np.sum((f[w, 3] + f[w, 4]) * area) / area.sum()))
+
+
+@pytest.mark.parametrize("model,T,seed", [(D.WALL_SPECULAR, 1.0, 3), (D.WALL_SPECULAR, 0.6, 5),
+                                          (D.WALL_DIFFUSE, 1.0, 3)])
+def test_wall_pressure_is_the_gas_pressure(model, T, seed):
+    """Kinetic pressure on the walls of a gas at rest in equilibrium is
+    p = n k T: in the wall output's unit n_ref k T_ref, rho T.  With specular
+    walls the gas keeps its sample temperature and no energy crosses the
+    wall (q = 0 exactly); a diffuse wall at T_w = T brings the gas to T_w."""
+    steps = 60
+    m, p, o, rho, t_sample = _box_wall_run(model, T, seed, steps)
+    pw, q = _wall_means(m, o.surface_fields(steps))
+    t_gas = t_sample if model == D.WALL_SPECULAR else T
+    print(f"[wall] model={model} T={T} p/(rho T)={pw / (rho * t_gas):.4f} q={q:.2e}")
+    assert abs(pw / (rho * t_gas) - 1.0) < 0.01
+    if model == D.WALL_SPECULAR:
+        assert q == 0.0
+    else:
+        assert abs(q) < 0.01 * pw
+    # planted: the same tallies turned into fields with Neff halved (or any
+    # wrong time, area or density scale) miss the gas pressure
+    q2 = p.copy()
+    q2.Neff *= 0.5
+    bad = O.Oracle(m, q2, seed=5)
+    pw_bad, _ = _wall_means(m, bad.surface_fields(steps, o.surface()))
+    assert abs(pw_bad / (rho * t_gas) - 1.0) > 0.2
