@@ -22,6 +22,7 @@ MACRO_LOCAL, MACRO_RECORD, MACRO_FINAL, MACRO_OUTPUT = 0, 1, 2, 3
 WALL_DIFFUSE, WALL_SPECULAR = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 TAG_INTERFACE, TAG_INLET, TAG_OUTLET, TAG_WALL = 3, 4, 5, 6
+ORDER_MORTON = 1
 MACRO_NAMES = ["rho", "ux", "uy", "uz", "T", "sxx", "sxy", "syy", "syz", "szz", "szx",
                "qx", "qy", "qz", "Tr", "qrx", "qry", "qrz"]
 
@@ -77,7 +78,7 @@ TUNE_KEEP = -2147483648
 
 class Tuning(C.Structure):
     _fields_ = [("wide_min", C.c_int32), ("wide_grid", C.c_int32), ("move_chunk", C.c_int32),
-                ("graph", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("graph", C.c_int32), ("sort_map", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class CaseInfo(C.Structure):
@@ -116,7 +117,7 @@ EXPORTED = [
     "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
     "dsmc_device_math_check", "dsmc_device_accept_check", "dsmc_ns_apply", "dsmc_ns_apply_noalloc", "dsmc_write_fluent_cells", "dsmc_write_fluent_walls",
-    "dsmc_mesh_read_cas", "dsmc_mesh_write_cas",
+    "dsmc_mesh_read_cas", "dsmc_mesh_write_cas", "dsmc_mesh_renumber",
     "dsmc_copy_probe", "dsmc_last_error",
 ]
 
@@ -227,6 +228,15 @@ class Mesh:
         h = C.c_void_p()
         check(lib().dsmc_mesh_read_cas(path.encode(), C.c_double(scale), C.byref(h)), "mesh_read_cas")
         return Mesh(h)
+
+    def renumber(self, order: int = 1) -> tuple["Mesh", np.ndarray]:
+        """The mesh with its cells in Morton order of their centroids, and
+        new_to_old: per-cell results r of the renumbered mesh map back to
+        this mesh's order as r_mesh[new_to_old] = r."""
+        perm = np.zeros(self.ncell, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().dsmc_mesh_renumber(self.h, int(order), C.byref(h), _ptr(perm, C.c_int32)), "mesh_renumber")
+        return Mesh(h), perm
 
     def write_cas(self, path: str):
         check(lib().dsmc_mesh_write_cas(C.byref(self.view), path.encode()), "mesh_write_cas")
@@ -547,11 +557,11 @@ class Engine:
         return b.value
 
     def set_tuning(self, wide_min: int | None = None, wide_grid: int | None = None,
-                   move_chunk: int | None = None, graph: int | None = None):
+                   move_chunk: int | None = None, graph: int | None = None, sort_map: int | None = None):
         """Launch tuning; results are bit-identical for every setting."""
         t = Tuning()
         for f, v in (("wide_min", wide_min), ("wide_grid", wide_grid), ("move_chunk", move_chunk),
-                     ("graph", graph)):
+                     ("graph", graph), ("sort_map", sort_map)):
             setattr(t, f, TUNE_KEEP if v is None else int(v))
         check(lib().dsmc_engine_set_tuning(self.h, C.byref(t)), "set_tuning")
 
@@ -584,13 +594,15 @@ class Engine:
         used = C.c_int32()
         check(lib().dsmc_autotune(self.h, int(first_step), int(steps_per_trial), C.byref(t), C.byref(used)),
               "autotune")
-        tuned = {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk, "graph": t.graph}
+        tuned = {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk, "graph": t.graph,
+                 "sort_map": t.sort_map}
         return tuned, used.value
 
     def tuning(self) -> dict:
         t = Tuning()
         check(lib().dsmc_engine_get_tuning(self.h, C.byref(t)), "get_tuning")
-        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk, "graph": t.graph}
+        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk, "graph": t.graph,
+                "sort_map": t.sort_map}
 
 
 class _Borrowed(Engine):

@@ -123,6 +123,18 @@ int dsmc_mesh_box_hexes(int32_t n, int32_t wall_tag, dsmc_mesh **out);
  * (0..0.3) perturbs interior nodes, seeded by `seed`. */
 int dsmc_mesh_capsule(int32_t a, int32_t c, int32_t layers, double diameter, int32_t tets,
                       double jitter, uint64_t seed, dsmc_mesh **out);
+/* The same mesh with its cells renumbered along a space-filling curve
+ * through their centroids (DSMC_ORDER_MORTON: Morton order of the centroids
+ * quantised to 2^21 steps over the bounding cube), so that neighbouring
+ * cells sit near each other in memory.  The reference renumbers its mesh
+ * too, after partitioning (reMesh, src/meshImport.cpp:1760).  Nodes and
+ * faces keep their order; every per-cell quantity (volume, centroid, face
+ * slots) is that of the original cell, bit for bit.  new_to_old (ncell, may
+ * be NULL) receives the original index of each new cell: per-cell outputs of
+ * a renumbered run map back to mesh order as out_mesh[new_to_old[c]] =
+ * out[c]. */
+#define DSMC_ORDER_MORTON 1
+int dsmc_mesh_renumber(const dsmc_mesh *m, int32_t order, dsmc_mesh **out, int32_t *new_to_old);
 int dsmc_mesh_get_view(const dsmc_mesh *m, dsmc_mesh_view *view);
 void dsmc_mesh_destroy(dsmc_mesh *m);
 
@@ -485,6 +497,9 @@ int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
  *              (steps numbered on the device; not while profiling or
  *              logging); 0: every step is launched kernel by kernel
  *              (default 1)
+ *   sort_map   1: the regroup's tiles are cut into one contiguous range per
+ *              XCD, so that tiles of neighbouring cells share an L2; 0:
+ *              tile = workgroup index (default 0)
  * Fields set to DSMC_TUNE_KEEP are left unchanged. */
 #define DSMC_TUNE_KEEP (-2147483647 - 1)
 typedef struct dsmc_tuning {
@@ -492,12 +507,13 @@ typedef struct dsmc_tuning {
   int32_t wide_grid;
   int32_t move_chunk;
   int32_t graph;
-  int32_t reserved[4];
+  int32_t sort_map;
+  int32_t reserved[3];
 } dsmc_tuning;
 int dsmc_engine_set_tuning(dsmc_engine *e, const dsmc_tuning *t);
 int dsmc_engine_get_tuning(dsmc_engine *e, dsmc_tuning *t);
-/* Autotuner (no reference counterpart): picks move_chunk, wide_min and
- * wide_grid for this case by timing the kernels on live steps first_step,
+/* Autotuner (no reference counterpart): picks move_chunk, wide_min,
+ * wide_grid and sort_map for this case by timing the kernels on live steps first_step,
  * first_step + 1, ... (the tuning never changes results, so these are
  * ordinary steps of the run), sets the best and returns it.  *steps_used
  * gets the number of steps run; continue the run from first_step +

@@ -75,6 +75,15 @@ def _case(kind, seed=3):
         p.NSS = 3
         p.surface_stats = 1
         parts = D.fill_freestream(mesh, p, ppc=8.0, seed=seed)
+    elif kind == "capsule_morton":
+        # the capsule with its cells renumbered in Morton order
+        mesh, _ = D.Mesh.capsule(a=6, c=12, layers=8, diameter=3.912).renumber(D.ORDER_MORTON)
+        p = D.params_apollo(monatomic=False)
+        p.Npinitial = 8
+        D.derive(mesh, p)
+        p.NSS = 3
+        p.surface_stats = 1
+        parts = D.fill_freestream(mesh, p, ppc=8.0, seed=seed)
     elif kind == "capsule_tets":
         mesh = D.Mesh.capsule(a=4, c=8, layers=5, diameter=3.912, tets=True, jitter=0.2, seed=5)
         p = D.params_apollo(monatomic=True)
@@ -164,7 +173,7 @@ def test_track_parity(kind, move_chunk):
                                            ("box_dense", None), ("box_dense", -1), ("capsule", 0),
                                            ("box_dense", 0), ("box_dense", 20), ("box_crowded", None),
                                            ("box_crowded", -1), ("box_packed", 0),
-                                           ("box_vhs", -1), ("box_vhs", 0)])
+                                           ("box_vhs", -1), ("box_vhs", 0), ("capsule_morton", None)])
 def test_full_step_parity(kind, wide_min):
     mesh, p, parts = _case(kind)
     print(f"[full_step_parity] {kind} wide_min={wide_min} cells={mesh.ncell} particles={parts.n}", flush=True)
