@@ -82,7 +82,8 @@ struct Counters {
   int32_t overflow;  // capacity exceeded (sticky)
   int32_t last_step;   // engine step being run (set by the step-begin kernel)
   int32_t park_count;  // particles parked at a wall by the move kernel this step
-  int32_t wide_n;      // cells queued for the one-wave-per-cell collide+sample kernel
+  int32_t wide_n;      // cells queued for the one-wave-per-cell kernel from the front (the largest)
+  int32_t wide_back;   // cells queued for it from the back of the queue (the others)
   unsigned long long emitted, particle_steps;
   TallySlot tally[TALLY_SLOTS];
 };
