@@ -19,12 +19,9 @@
 
 namespace dsmc {
 
-// One triangle of a cell's face cache (FaceTriCache, src/ProcessDSMC.h:109-116):
-// vertex A, edges e1 = B - A, e2 = C - A, the face it belongs to and its tag
-// (-1 for triangular faces, 1/2 for the halves of a split quad).  80 bytes.
-// One triangle of a cell's face (FaceTriCache entry): vertex A, edges
-// e1 = B - A, e2 = C - A, and two packed words so that the walk decides what
-// a crossing does without reading the face table:
+// One triangle of a cell's face cache (FaceTriCache, src/ProcessDSMC.h:109-116),
+// 80 bytes: vertex A, edges e1 = B - A, e2 = C - A, and two packed words so
+// that the walk decides what a crossing does without reading the face table:
 //   face_tag = face | tagcode << 29   (tagcode 0: triangle face, tag -1;
 //                                      1, 2: half 1 or 2 of a split quad)
 //   link     = the cell across the face, if the face is an interface with a
