@@ -3,7 +3,10 @@
 // Particle state is structure-of-arrays in HBM, grouped by cell after every
 // step: positions in FP64, velocities and Ir in VT (double, or float for the
 // mixed-precision engine), cell (int32) and a stable id (uint32).  Two copies
-// of every array exist; the regroup gathers from one into the other.
+// of every array exist.  The regroup gathers the velocity records from one
+// into the other; the position records stay where the move wrote them, and
+// the next move reads each through the regroup's permutation and writes it
+// back in cell order.
 //
 // All kernels read the live particle count from device memory, so a whole
 // step (and a whole run) is enqueued without host round trips.
@@ -119,8 +122,8 @@ struct CrossLog {
 
 // Particle records: position + cell + id (32 B) and velocity + Ir (32 B FP64,
 // 16 B mixed).  A record is read or written with two (one) 16-byte loads, so
-// the regroup moves each particle as two contiguous pieces instead of nine
-// scattered scalars, and collide/sample reads only the velocity records.
+// a particle moves as two contiguous pieces instead of nine scattered
+// scalars, and collide/sample reads only the velocity records.
 struct alignas(32) PosRec {
   double x, y, z;
   int32_t cell;
