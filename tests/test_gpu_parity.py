@@ -26,7 +26,7 @@ def _same_particles(g, c):
 
 
 def test_acceptance_estimate_within_its_band():
-    """The collision kernels decide u < c(cr v_rms)/consant_crm from an FP32
+    """The collision kernels decide u < c(cr v_rms)/(sigma c)max from an FP32
     estimate when u is outside a relative band of 2^-11 around it.  That is
     exact only if the estimate's error stays inside the band: checked here
     (to a quarter of the band) over speeds spanning the guarded range, for
@@ -373,6 +373,54 @@ def test_stagewise_matches_full_step():
     for k in a:
         assert a[k] == b[k], (k, a[k], b[k])
     assert np.array_equal(split.macro(D.MACRO_OUTPUT), full.macro(D.MACRO_OUTPUT), equal_nan=True)
+
+
+def test_irregular_stage_sequences_match_the_oracle():
+    """After a regroup the engine holds the particles' positions through the
+    regroup's permutation until the next move writes them in cell order.
+    Every path that needs them in cell order first must see the same
+    particles as the oracle: a regroup with no move (cache2chain without
+    advection), emission in a call of its own before the move, a download
+    between the regroup and the move, graph-replayed runs after that, and
+    the continuum coupling after a regroup."""
+    mesh, p, parts = _case("capsule")
+    print(f"[stage sequences] capsule cells={mesh.ncell} particles={parts.n}", flush=True)
+    eng = D.Engine(mesh, p, seed=21)
+    eng.upload(parts)
+    orc = O.Oracle(mesh, p, seed=21)
+    orc.set_particles(parts)
+
+    def both(s, mask=D.STAGE_ALL):
+        eng.step(s, mask)
+        orc.step(s, mask)
+
+    both(1)
+    both(2, D.STAGE_SORT)                       # regroup with no move
+    both(2, D.STAGE_COLLIDE | D.STAGE_SAMPLE)
+    _same_particles(eng.download(), orc.particles())
+    both(3, D.STAGE_EMIT)                       # emission on its own
+    both(3, D.STAGE_MOVE | D.STAGE_SORT)
+    _same_particles(eng.download(), orc.particles())  # positions put in cell order here
+    both(3, D.STAGE_COLLIDE | D.STAGE_SAMPLE)
+    both(4)
+    eng.run(5, 9)                               # graph replays after the download
+    for s in range(5, 10):
+        orc.step(s)
+    _same_particles(eng.download(), orc.particles())
+    for fam in (0, 1, 2):
+        assert np.array_equal(eng.accum(fam), orc.accum(fam)), fam
+    # the continuum coupling after a full step, whose regroup left the
+    # positions held through the permutation
+    both(10)
+    counts = eng.cell_scalars()["count"].astype(np.float64)
+    ns, acc, stop = _ns_fields(mesh, p, counts)
+    assert eng.ns_apply(11, ns, acc) == stop
+    assert orc.ns_apply(11, ns, acc) == stop
+    _same_particles(eng.download(), orc.particles())
+    both(11)
+    _same_particles(eng.download(), orc.particles())
+    for fam in (0, 1, 2):
+        assert np.array_equal(eng.accum(fam), orc.accum(fam)), fam
 
 
 def test_stage_order_guard():
