@@ -45,6 +45,12 @@ def alg_bytes(kernel: str, S: int, Sv: int, mtri: float, ppc: float) -> float:
     raise KeyError(kernel)
 
 
+def trial_steps(n_particles: int) -> int:
+    """Steps per autotuner trial: about 3e7 particle-steps, so that a trial on
+    a small case lasts long enough to rank the settings (2 to 8 steps)."""
+    return max(2, min(8, round(3e7 / max(n_particles, 1))))
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -90,7 +96,7 @@ def run_engine(mesh, p, parts, steps, warmup, seed=20261203, autotune=True, tuni
     if autotune:
         # set-up: the autotuner times live steps (results do not depend on the
         # tuning it picks), then the case restarts from the same particles
-        tuning, tune_steps = eng.autotune(1, 2)
+        tuning, tune_steps = eng.autotune(1, trial_steps(parts.n))
         log(f"[bench] autotune ({tune_steps} steps): {tuning}")
     # set-up: capture the step-pair graphs of both buffer parities (each run
     # starts with one eager step, which flips the parity), then restart from
@@ -245,7 +251,7 @@ def apollo_record(steps: int = 500, nss: int = 250, ppc: float = 20.0, seed: int
         q.precision = prec
         eng = D.Engine(mesh, q, seed=seed)
         eng.upload(parts)
-        tuning, _ = eng.autotune(1, 2)
+        tuning, _ = eng.autotune(1, trial_steps(parts.n))
         eng.upload(parts)
         eng.run(1, 3)
         eng.run(4, 6)
@@ -332,7 +338,7 @@ def validate_apollo(steps: int, ppc: float = 100.0, nss: int = 5000, nscheme: in
         parts = D.fill_freestream(mesh, p, ppc=ppc, seed=seed)
         eng = D.Engine(mesh, p, seed=seed)
         eng.upload(parts)
-        tuning, s = eng.autotune(1, 2)
+        tuning, s = eng.autotune(1, trial_steps(parts.n))
         s += 1
         t0 = time.perf_counter()
         while s <= steps:
@@ -406,7 +412,7 @@ def run_sweep(args):
     n = eng.fill_device([sp["ppc"] for sp in specs], seeds)
     log(f"[sweep] batch: {eng.mesh.ncell} cells, {n} particles, {eng.footprint() / 1e9:.1f} GB "
         f"({time.perf_counter() - t0:.1f} s)")
-    tuning, used = eng.autotune(1, 2)
+    tuning, used = eng.autotune(1, trial_steps(n))
     s = used + 1
     if args.warmup:
         eng.run(s, s + args.warmup - 1)
