@@ -76,9 +76,14 @@ class KernelTimes(C.Structure):
 TUNE_KEEP = -2147483648
 
 
+def _tuning_dict(t) -> dict:
+    return {f: getattr(t, f) for f in ("wide_min", "wide_grid", "move_chunk", "graph", "sort_map", "coll_block")}
+
+
 class Tuning(C.Structure):
     _fields_ = [("wide_min", C.c_int32), ("wide_grid", C.c_int32), ("move_chunk", C.c_int32),
-                ("graph", C.c_int32), ("sort_map", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("graph", C.c_int32), ("sort_map", C.c_int32), ("coll_block", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class CaseInfo(C.Structure):
@@ -557,11 +562,12 @@ class Engine:
         return b.value
 
     def set_tuning(self, wide_min: int | None = None, wide_grid: int | None = None,
-                   move_chunk: int | None = None, graph: int | None = None, sort_map: int | None = None):
+                   move_chunk: int | None = None, graph: int | None = None, sort_map: int | None = None,
+                   coll_block: int | None = None):
         """Launch tuning; results are bit-identical for every setting."""
         t = Tuning()
         for f, v in (("wide_min", wide_min), ("wide_grid", wide_grid), ("move_chunk", move_chunk),
-                     ("graph", graph), ("sort_map", sort_map)):
+                     ("graph", graph), ("sort_map", sort_map), ("coll_block", coll_block)):
             setattr(t, f, TUNE_KEEP if v is None else int(v))
         check(lib().dsmc_engine_set_tuning(self.h, C.byref(t)), "set_tuning")
 
@@ -594,15 +600,12 @@ class Engine:
         used = C.c_int32()
         check(lib().dsmc_autotune(self.h, int(first_step), int(steps_per_trial), C.byref(t), C.byref(used)),
               "autotune")
-        tuned = {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk, "graph": t.graph,
-                 "sort_map": t.sort_map}
-        return tuned, used.value
+        return _tuning_dict(t), used.value
 
     def tuning(self) -> dict:
         t = Tuning()
         check(lib().dsmc_engine_get_tuning(self.h, C.byref(t)), "get_tuning")
-        return {"wide_min": t.wide_min, "wide_grid": t.wide_grid, "move_chunk": t.move_chunk, "graph": t.graph,
-                "sort_map": t.sort_map}
+        return _tuning_dict(t)
 
 
 class _Borrowed(Engine):

@@ -500,6 +500,8 @@ int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
  *   sort_map   1: the regroup's tiles are cut into one contiguous range per
  *              XCD, so that tiles of neighbouring cells share an L2; 0:
  *              tile = workgroup index (default 0)
+ *   coll_block threads per workgroup of the thread-per-cell collide+sample
+ *              kernel (one cell per thread): 64, 128 or 256 (default 64)
  * Fields set to DSMC_TUNE_KEEP are left unchanged. */
 #define DSMC_TUNE_KEEP (-2147483647 - 1)
 typedef struct dsmc_tuning {
@@ -508,12 +510,13 @@ typedef struct dsmc_tuning {
   int32_t move_chunk;
   int32_t graph;
   int32_t sort_map;
-  int32_t reserved[3];
+  int32_t coll_block;
+  int32_t reserved[2];
 } dsmc_tuning;
 int dsmc_engine_set_tuning(dsmc_engine *e, const dsmc_tuning *t);
 int dsmc_engine_get_tuning(dsmc_engine *e, dsmc_tuning *t);
 /* Autotuner (no reference counterpart): picks move_chunk, wide_min,
- * wide_grid and sort_map for this case by timing the kernels on live steps first_step,
+ * wide_grid, coll_block and sort_map for this case by timing the kernels on live steps first_step,
  * first_step + 1, ... (the tuning never changes results, so these are
  * ordinary steps of the run), sets the best and returns it.  *steps_used
  * gets the number of steps run; continue the run from first_step +

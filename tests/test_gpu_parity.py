@@ -350,6 +350,43 @@ def test_wide_matches_thread_path(kind, prec):
     assert sa["collisions"] == sb["collisions"] > 0
 
 
+@pytest.mark.parametrize("kind", ["capsule", "box_crowded"])
+def test_collide_workgroup_size_does_not_change_results(kind):
+    """The thread-per-cell collide+sample kernel at 64, 128 and 256 threads
+    per workgroup (the autotuner's coll_block setting): particles, cell
+    scalars, accumulators and counters bit-identical, on the capsule (cells
+    of every size, crowded ones left to the wide kernel) and on a box whose
+    cells all go through the thread path."""
+    mesh, p, parts = _case(kind)
+    print(f"[coll_block] {kind} cells={mesh.ncell} particles={parts.n}", flush=True)
+    out = []
+    for wg in (64, 128, 256):
+        eng = D.Engine(mesh, p, seed=37)
+        eng.set_tuning(coll_block=wg, wide_min=-1 if kind == "box_crowded" else 96)
+        assert eng.tuning()["coll_block"] == wg
+        eng.upload(parts)
+        eng.run(1, 5)
+        out.append((eng.download(), eng.cell_scalars(), [eng.accum(f) for f in (0, 1, 2)], eng.stats()))
+        eng.close()
+    q0, c0, a0, s0 = out[0]
+    for q, c, a, st in out[1:]:
+        _same_particles(q0, q)
+        for k in c0:
+            assert np.array_equal(c0[k], c[k]), k
+        for fa, fb in zip(a0, a):
+            assert np.array_equal(fa, fb)
+        assert st == s0
+    assert s0["collisions"] > 0
+
+
+def test_collide_workgroup_size_is_checked():
+    mesh, p, parts = _case("capsule")
+    eng = D.Engine(mesh, p, seed=1)
+    with pytest.raises(RuntimeError, match="coll_block"):
+        eng.set_tuning(coll_block=100)
+    eng.close()
+
+
 def test_stagewise_matches_full_step():
     """Driving a step stage by stage, the way the reference driver calls
     preprocesseffquad, advection, cache2chain, collisionDSMC and
@@ -449,6 +486,7 @@ def test_autotune_steps_are_ordinary_steps():
     tuned.upload(parts)
     tuning, used = tuned.autotune(1, 1)
     assert used >= 10 and tuning["move_chunk"] in (512, 1024, 2048, 4096, 8192)
+    assert tuning["coll_block"] in (64, 128, 256)
     last = used + 2
     tuned.run(used + 1, last)
     plain = D.Engine(mesh, p, seed=9)
