@@ -373,3 +373,74 @@ def test_wall_pressure_is_the_gas_pressure(model, T, seed):
     bad = O.Oracle(m, q2, seed=5)
     pw_bad, _ = _wall_means(m, bad.surface_fields(steps, o.surface()))
     assert abs(pw_bad / (rho * t_gas) - 1.0) > 0.2
+
+
+def _tet_nodes(a, c):
+    """The four nodes of tet cell c (from its faces), in a fixed order."""
+    nodes = []
+    for f in a["cell_faces"][c][: a["cell_nface"][c]]:
+        for v in a["face_nodes"][f]:
+            if v >= 0 and v not in nodes:
+                nodes.append(int(v))
+    assert len(nodes) == 4
+    return a["node_xyz"][nodes]
+
+
+def test_initial_fill_closed_forms():
+    """The initial free-stream fill (src/MeshparticalInitial.cpp:392-721 with
+    the per-cell count rounding of src/meshImport.cpp:1409-1421), product
+    host code that the device fill reproduces bit for bit, against what
+    the sampling law fixes:
+    * each cell gets floor(N_c) or floor(N_c) + 1 particles, N_c = its volume
+      share of ppc x ncell, and the round-ups add up to the fractional parts
+      (a plain floor, or a share of the cell count instead of the volume,
+      fails);
+    * every particle lies in its tet, and its barycentric coordinates have
+      the uniform (Dirichlet(1,1,1,1)) mean 1/4 and variance 3/80;
+    * velocities are the drifting Maxwellian: mean u_inf, variance T/2 per
+      component (sqrt(T) for sqrt(T/2) fails by a factor 2), rotational
+      energy exponential with mean T/2 for the diatomic gas."""
+    m = D.Mesh.box_tets(5)
+    # jitter-free box tets have equal volumes; use the capsule for the counts
+    cap = D.Mesh.capsule(a=6, c=12, layers=8, diameter=3.912)
+    p = D.derive(cap, D.params_apollo(monatomic=False))
+    ppc = 30.0
+    parts = D.fill_freestream(cap, p, ppc=ppc, seed=11)
+    ac = cap.arrays()
+    vol = ac["cell_volume"]
+    ns = vol * ppc * cap.ncell / vol.sum()
+    cnt = np.bincount(parts.cell, minlength=cap.ncell)
+    up = cnt - np.floor(ns)
+    assert set(np.unique(up)) <= {0.0, 1.0}
+    frac = ns - np.floor(ns)
+    sd = math.sqrt(float(np.sum(frac * (1 - frac))))
+    print(f"[fill] {parts.n} particles; round-ups {up.sum():.0f} against {frac.sum():.1f} +- {sd:.1f}")
+    assert abs(up.sum() - frac.sum()) < 5 * sd + 1
+    # velocities: drifting Maxwellian and exponential rotational energy
+    T = p.T_inf
+    n = parts.n
+    for comp, u in zip(("vx", "vy", "vz"), p.u_inf):
+        v = getattr(parts, comp) - u
+        assert abs(v.mean()) < 5 * math.sqrt(T / 2 / n), comp
+        assert abs(v.var() / (T / 2) - 1) < 5 * math.sqrt(2 / n), comp
+    ir = parts.ir
+    assert ir.min() >= 0.0
+    assert abs(ir.mean() / (T / 2) - 1) < 5 / math.sqrt(n)
+    assert abs(ir.var() / (T / 2) ** 2 - 1) < 10 / math.sqrt(n)
+    # positions: inside the tet, uniformly
+    pb = D.derive(m, D.params_box(0.05))
+    box = D.fill_freestream(m, pb, ppc=200.0, temperature=1.0, seed=4)
+    a = m.arrays()
+    lam = np.empty((box.n, 4))
+    xyz = np.stack([box.x, box.y, box.z], axis=1)
+    for c in range(m.ncell):
+        sel = np.flatnonzero(box.cell == c)
+        v = _tet_nodes(a, c)
+        mat = np.column_stack([v[1] - v[0], v[2] - v[0], v[3] - v[0]])
+        l123 = np.linalg.solve(mat, (xyz[sel] - v[0]).T).T
+        lam[sel, 1:] = l123
+        lam[sel, 0] = 1.0 - l123.sum(axis=1)
+    assert lam.min() > -1e-12
+    nb = box.n
+    assert np.all(np.abs(lam.mean(axis=0) - 0.25) < 5 * math.sqrt(3 / 80 / nb))
+    assert np.all(np.abs(lam.var(axis=0) / (3 / 80) - 1) < 0.05)
