@@ -527,7 +527,8 @@ def main():
                     help="requests admitted together share one batch engine, up to this many")
     ap.add_argument("--serve-hw-queues", type=int, default=16,
                     help="HIP hardware queues of the serving process (sets GPU_MAX_HW_QUEUES; 0 keeps the environment's)")
-    ap.add_argument("--serve-scale", type=float, default=0.1, help="multiplies every request's cell count")
+    ap.add_argument("--serve-scale", type=float, default=1.0,
+                    help="multiplies every request's cell count (1: the sweep's 100k-1M-cell cases)")
     ap.add_argument("--sweep-cases", type=int, default=32)
     ap.add_argument("--sweep-scale", type=float, default=1.0, help="multiplies every sweep case's cell count")
     ap.add_argument("--no-sequential", action="store_true", help="sweep: skip the one-case-at-a-time comparison")
