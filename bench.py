@@ -462,11 +462,28 @@ def run_serve(args):
     specs = [SW.case_spec(i, args.serve_scale) for i in range(args.serve_requests)]
     built = [SW.build_case(sp, D.PREC_MIXED, steps=args.serve_steps) for sp in specs]
     log(f"[serve] {len(specs)} requests prepared in {time.perf_counter() - t_set:.1f} s")
+    tuning = None
+    if args.serve_tuning == "auto":
+        # set-up, untimed: the autotuner's pick for an engine like those the
+        # server builds (the first requests' cases in one batch engine), given
+        # to every engine the server builds; results do not depend on it
+        k = max(1, min(args.serve_batch, len(specs)))
+        tb = [SW.build_case(sp, D.PREC_MIXED, steps=args.serve_steps) for sp in specs[:k]]
+        seeds = [1000 + i for i in range(k)]
+        want = sum(sp["ppc"] * m.ncell for sp, (m, _) in zip(specs, tb))
+        cap = int(1.25 * want) + 2_000_000 * k
+        eng = (D.Engine.batch([m for m, _ in tb], [p for _, p in tb], seeds, capacity=cap) if k > 1 else
+               D.Engine(tb[0][0], tb[0][1], seed=seeds[0], capacity=cap))
+        n = eng.fill_device([sp["ppc"] for sp in specs[:k]], seeds)
+        tuning, used = eng.autotune(1, trial_steps(n))
+        eng.close()
+        del tb
+        log(f"[serve] autotune on the first {k} requests' cases ({n} particles, {used} steps): {tuning}")
     rng = np.random.default_rng(20261203)
     arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(specs)))
     arrivals -= arrivals[0]
     srv = D.Server(budget_bytes=0, chunk_steps=args.serve_chunk, max_running=args.serve_max_running,
-                   batching=args.serve_batch)
+                   batching=args.serve_batch, tuning=tuning)
     t0 = time.perf_counter()
     nxt, ids, last_log = 0, [], t0
     while True:
@@ -503,6 +520,8 @@ def run_serve(args):
                                + (f", up to {args.serve_batch} requests per engine" if args.serve_batch > 1 else
                                   ", one engine per request")
                                + f", {os.environ.get('GPU_MAX_HW_QUEUES', '4')} hardware queues"
+                               + (f", launch settings {tuning} (autotuned in set-up)" if tuning else
+                                  ", engine default launch settings")
                                + ", mixed precision",
                    "particles_initial": int(sum(i["particles"] for i in infos)), "parallelism": "single GPU"},
         "latency_s": {"p50": round(p50, 3), "p99": round(p99, 3), "max": round(float(lat.max()), 3),
@@ -527,6 +546,8 @@ def main():
                     help="requests admitted together share one batch engine, up to this many")
     ap.add_argument("--serve-hw-queues", type=int, default=16,
                     help="HIP hardware queues of the serving process (sets GPU_MAX_HW_QUEUES; 0 keeps the environment's)")
+    ap.add_argument("--serve-tuning", choices=["auto", "default"], default="auto",
+                    help="auto: autotune once in set-up and give the pick to every engine the server builds")
     ap.add_argument("--serve-scale", type=float, default=1.0,
                     help="multiplies every request's cell count (1: the sweep's 100k-1M-cell cases)")
     ap.add_argument("--sweep-cases", type=int, default=32)

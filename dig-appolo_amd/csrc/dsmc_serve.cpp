@@ -27,6 +27,7 @@
 #include "dsmc_engine.h"
 #include "dsmc_internal.h"
 
+using dsmc::check_tuning;
 using dsmc::set_error;
 
 namespace {
@@ -71,6 +72,8 @@ struct dsmc_server {
   int32_t auto_release = 1;
   int32_t max_running = 0;  // 0: no limit
   int32_t max_cases = 1;    // requests per engine
+  bool tuned = false;       // `tuning` goes to every engine built (else the engine defaults)
+  dsmc_tuning tuning{};
   std::vector<Request> reqs;
   std::vector<Group> groups;
   std::deque<size_t> queue;  // indices of queued requests, arrival order
@@ -151,6 +154,7 @@ int start_group(dsmc_server* s, Group& g) {
   int rc = k == 1 ? dsmc_engine_create(&views[0], &ps[0], seeds[0], cap, &g.eng)
                   : dsmc_batch_create((int32_t)k, views.data(), ps.data(), seeds.data(), cap, &g.eng);
   if (rc) return rc;
+  if (s->tuned && (rc = dsmc_engine_set_tuning(g.eng, &s->tuning))) return rc;
   for (size_t m : g.members) {
     Request& r = s->reqs[m];
     dsmc_mesh_destroy(r.mesh);
@@ -323,6 +327,18 @@ int dsmc_server_set_max_running(dsmc_server* s, int32_t max_running) {
 int dsmc_server_set_batching(dsmc_server* s, int32_t max_cases) {
   if (!s || max_cases < 1) return set_error(DSMC_ERR_ARG, "dsmc_server_set_batching: bad arguments");
   s->max_cases = max_cases;
+  return DSMC_OK;
+}
+
+int dsmc_server_set_tuning(dsmc_server* s, const dsmc_tuning* t) {
+  if (!s) return set_error(DSMC_ERR_ARG, "dsmc_server_set_tuning: null server");
+  if (!t) {
+    s->tuned = false;
+    return DSMC_OK;
+  }
+  if (int rc = check_tuning(t, "dsmc_server_set_tuning")) return rc;
+  s->tuning = *t;
+  s->tuned = true;
   return DSMC_OK;
 }
 

@@ -118,7 +118,8 @@ EXPORTED = [
     "dsmc_surface_fields", "dsmc_stats_get", "dsmc_profile_enable", "dsmc_profile_get", "dsmc_last_run_ms",
     "dsmc_batch_create", "dsmc_case_info", "dsmc_case_surface_fields", "dsmc_fill_device",
     "dsmc_run_async", "dsmc_run_query", "dsmc_run_on_stream", "dsmc_engine_estimate", "dsmc_server_create", "dsmc_server_destroy",
-    "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_set_batching", "dsmc_server_request",
+    "dsmc_server_submit", "dsmc_server_pump", "dsmc_server_set_max_running", "dsmc_server_set_batching",
+    "dsmc_server_set_tuning", "dsmc_server_request",
     "dsmc_server_engine", "dsmc_server_release",
     "dsmc_engine_footprint", "dsmc_engine_set_tuning", "dsmc_engine_get_tuning", "dsmc_autotune",
     "dsmc_device_math_check", "dsmc_device_accept_check", "dsmc_ns_apply", "dsmc_ns_apply_noalloc", "dsmc_write_fluent_cells", "dsmc_write_fluent_walls",
@@ -622,7 +623,7 @@ class Server:
     """Serving loop with continuous batching (see dsmc_server_create)."""
 
     def __init__(self, budget_bytes: int = 0, chunk_steps: int = 50, auto_release: bool = True,
-                 max_running: int = 0, batching: int = 1):
+                 max_running: int = 0, batching: int = 1, tuning: dict | None = None):
         self.h = C.c_void_p()
         check(lib().dsmc_server_create(C.c_int64(budget_bytes), int(chunk_steps), int(auto_release),
                                        C.byref(self.h)), "server_create")
@@ -631,6 +632,21 @@ class Server:
             check(lib().dsmc_server_set_max_running(self.h, int(max_running)), "server_set_max_running")
         if batching != 1:
             check(lib().dsmc_server_set_batching(self.h, int(batching)), "server_set_batching")
+        if tuning is not None:
+            self.set_tuning(tuning)
+
+    def set_tuning(self, tuning: dict | None):
+        """Launch settings (Engine.set_tuning's keys; missing keys keep the
+        engine default) for every engine built from now on; None restores
+        the defaults."""
+        if tuning is None:
+            check(lib().dsmc_server_set_tuning(self.h, None), "server_set_tuning")
+            return
+        t = Tuning()
+        for f in ("wide_min", "wide_grid", "move_chunk", "graph", "sort_map", "coll_block"):
+            v = tuning.get(f)
+            setattr(t, f, TUNE_KEEP if v is None else int(v))
+        check(lib().dsmc_server_set_tuning(self.h, C.byref(t)), "server_set_tuning")
 
     def close(self):
         if getattr(self, "h", None):

@@ -515,6 +515,12 @@ typedef struct dsmc_tuning {
 } dsmc_tuning;
 int dsmc_engine_set_tuning(dsmc_engine *e, const dsmc_tuning *t);
 int dsmc_engine_get_tuning(dsmc_engine *e, dsmc_tuning *t);
+/* Serving (above): launch settings for every engine the server builds from
+ * now on (dsmc_engine_set_tuning's fields; DSMC_TUNE_KEEP keeps a default),
+ * e.g. the autotuner's pick for a representative case of the stream; null
+ * restores the engine defaults.  Engines already running keep theirs.  No
+ * reference counterpart. */
+int dsmc_server_set_tuning(dsmc_server *s, const dsmc_tuning *t);
 /* Autotuner (no reference counterpart): picks move_chunk, wide_min,
  * wide_grid, coll_block and sort_map for this case by timing the kernels on live steps first_step,
  * first_step + 1, ... (the tuning never changes results, so these are

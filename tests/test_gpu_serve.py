@@ -106,6 +106,43 @@ def test_batched_requests_match_alone():
     srv.close()
 
 
+def test_server_tuning_reaches_its_engines():
+    """dsmc_server_set_tuning: every engine the server builds afterwards gets
+    the launch settings (checked when they are set), and the results stay
+    bit-identical to each case run alone with the engine defaults."""
+    steps, seeds = 12, [801, 802]
+    specs = [SW.case_spec(i, 0.02) for i in (11, 12)]
+    want = {"wide_min": 32, "wide_grid": 512, "move_chunk": 512, "graph": 0, "sort_map": 1, "coll_block": 128}
+    srv = D.Server(budget_bytes=0, chunk_steps=5, auto_release=False, batching=2)
+    with pytest.raises(RuntimeError, match="coll_block"):
+        srv.set_tuning(dict(want, coll_block=100))
+    srv.set_tuning(want)
+    ids = []
+    for sp, sd in zip(specs, seeds):
+        m, p = SW.build_case(sp, D.PREC_MIXED, steps=steps)
+        ids.append(srv.submit(m, p, seed=sd, ppc=sp["ppc"], steps=steps))
+    while srv.pump()[2] < len(ids):
+        pass
+    for rid, sp, sd in zip(ids, specs, seeds):
+        info = srv.request(rid)
+        assert info["state"] == D.REQ_DONE, info["error"]
+        m, p = SW.build_case(sp, D.PREC_MIXED, steps=steps)
+        served = srv.engine(rid, p)
+        got = served.tuning()
+        print(f"[serve tuning] request {rid}: {got}", flush=True)
+        assert {k: got[k] for k in want} == want
+        alone = D.Engine(m, p, seed=sd)
+        alone.fill_device(sp["ppc"], sd)
+        alone.run(1, steps)
+        k = info["kase"]
+        _same(served.case_particles(served.download(), k), alone.download())
+        alone.close()
+    for rid in ids:
+        srv.release(rid)
+    srv.set_tuning(None)
+    srv.close()
+
+
 def test_running_cap_holds_requests_in_the_queue():
     """With at most one request running, three submitted requests run one
     after another in arrival order (each admitted only after the previous
