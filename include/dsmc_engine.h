@@ -488,11 +488,11 @@ int dsmc_engine_footprint(dsmc_engine *e, int64_t *bytes);
  * setting; they only move work between kernels.
  *   wide_min   cells with more particles than this are collided and sampled
  *              by one wavefront per cell instead of one thread; < 0: never
- *              (default 64)
+ *              (default 128)
  *   wide_grid  workgroups of that one-wave-per-cell kernel (default 1024)
  *   move_chunk particles per workgroup of the move kernel, whose lanes take
  *              the chunk's particles one after another (64..1048576,
- *              default 2048)
+ *              default 4096)
  *   graph      1: dsmc_run replays pairs of steps captured as a HIP graph
  *              (steps numbered on the device; not while profiling or
  *              logging); 0: every step is launched kernel by kernel
