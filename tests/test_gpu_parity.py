@@ -167,6 +167,95 @@ def test_track_parity(kind, move_chunk):
         assert gs[k] == cs[k], (k, gs[k], cs[k])
 
 
+def _degenerate_starts(mesh, seed=7):
+    """Particles of every cell started where the face walk is degenerate: on
+    each of the cell's nodes, face centroids and edge midpoints (on a face,
+    so the first exit can be at t = 0, shared with neighbouring cells), and
+    at its centroid at rest, moving parallel to a face and moving along the
+    axes (the box's faces are axis planes).  Velocities elsewhere are
+    Maxwellian at T = 1/2 per component."""
+    a = mesh.arrays()
+    rng = np.random.default_rng(seed)
+    X = a["node_xyz"]
+    pos, vel, cells = [], [], []
+    for c in range(mesh.ncell):
+        faces = a["cell_faces"][c][: a["cell_nface"][c]]
+        nodes, edges = [], set()
+        for f in faces:
+            fn = [int(v) for v in a["face_nodes"][f] if v >= 0]
+            for k, v in enumerate(fn):
+                if v not in nodes:
+                    nodes.append(v)
+                w = fn[(k + 1) % len(fn)]
+                edges.add((min(v, w), max(v, w)))
+        pts = [X[v] for v in nodes]
+        pts += [X[[int(v) for v in a["face_nodes"][f] if v >= 0]].mean(axis=0) for f in faces]
+        pts += [0.5 * (X[i] + X[j]) for i, j in sorted(edges)]
+        for q in pts:
+            pos.append(q)
+            vel.append(rng.normal(0.0, np.sqrt(0.5), 3))
+            cells.append(c)
+        # a direction in the plane of face 0: across it a face whose normal
+        # is not parallel to face 0's
+        n0 = a["face_normal"][faces[0]]
+        t = max((np.cross(n0, a["face_normal"][f]) for f in faces[1:]), key=np.linalg.norm)
+        assert np.linalg.norm(t) > 1e-3
+        for v in (np.zeros(3), t / np.linalg.norm(t), np.array([1.0, 0.0, 0.0]), np.array([0.0, -0.7, 0.0])):
+            pos.append(a["cell_centroid"][c])
+            vel.append(v)
+            cells.append(c)
+    pos, vel = np.array(pos), np.array(vel)
+    P = D.Particles(len(pos))
+    P.x[:], P.y[:], P.z[:] = pos[:, 0], pos[:, 1], pos[:, 2]
+    P.vx[:], P.vy[:], P.vz[:] = vel[:, 0], vel[:, 1], vel[:, 2]
+    P.cell[:] = cells
+    P.id[:] = np.arange(P.n, dtype=np.uint32)
+    return P
+
+
+@pytest.mark.parametrize("kind", ["box_tets", "box_hex", "capsule"])
+def test_track_parity_degenerate_starts(kind):
+    """Walks that start on nodes, edges and faces (exits at t = 0, rays in a
+    face's plane, particles at rest) take the reference's decisions bit for
+    bit: the same crossings, wall hits and drops as the oracle.  Most of
+    these particles are dropped on their first visit, as the reference drops
+    them: a walk whose every hit has t <= 1e-14 finds no exit, and neither
+    does a particle at rest (det = 0 on every face)."""
+    if kind == "capsule":
+        mesh = D.Mesh.capsule(a=4, c=8, layers=4, diameter=3.912)
+        p = D.params_apollo(monatomic=True)
+        p.Npinitial = 4
+    else:
+        mesh = D.Mesh.box_tets(4) if kind == "box_tets" else D.Mesh.box_hexes(4)
+        p = D.params_box(0.05)
+        p.Npinitial = 10
+        if kind == "box_hex":
+            p.wall_model = D.WALL_DIFFUSE
+    D.derive(mesh, p)
+    parts = _degenerate_starts(mesh)
+    rest = np.flatnonzero((parts.vx == 0.0) & (parts.vy == 0.0) & (parts.vz == 0.0))
+    print(f"[track_degenerate] {kind} cells={mesh.ncell} particles={parts.n} at rest={rest.size}", flush=True)
+    eng = D.Engine(mesh, p, seed=11, capacity=2 * parts.n)
+    eng.upload(parts)
+    orc = O.Oracle(mesh, p, seed=11)
+    orc.set_particles(parts)
+    eng.log_enable(1_000_000)
+    orc.log_enable(True)
+    for s in (1, 2, 3):
+        eng.step(s, D.STAGE_MOVE | D.STAGE_SORT)
+        orc.step(s, D.STAGE_MOVE | D.STAGE_SORT)
+    ge, ce = _sorted_log(eng.log_get()), _sorted_log(orc.log_get())
+    assert len(ge) == len(ce) and len(ge) > 0
+    assert np.array_equal(ge, ce)
+    g = eng.download()
+    _same_particles(g, orc.particles())
+    gs, cs = eng.stats(), orc.stats()
+    for k in ("particles", "removed", "dropped", "wall_hits", "crossings"):
+        assert gs[k] == cs[k], (k, gs[k], cs[k])
+    assert gs["dropped"] > 0 and gs["crossings"] > 0
+    assert not np.isin(parts.id[rest], g.id).any()
+
+
 # wide_min: None = default routing; 0 = every cell one wave per cell; -1 =
 # every cell one thread per cell
 @pytest.mark.parametrize("kind,wide_min", [("box_tets", None), ("capsule", None), ("capsule_tets", None),
