@@ -124,6 +124,15 @@ def _case(kind, seed=3):
         p.Npinitial = 2000
         D.derive(mesh, p)
         parts = D.fill_freestream(mesh, p, ppc=2000.0, temperature=1.0, seed=seed)
+    elif kind == "box_sparse":
+        # 1.5 particles per cell: empty cells, singletons and pairs (no NTC
+        # candidates, one candidate), and cells emptied and refilled between
+        # steps, in the moment sums, the windows and the sparse-cell records
+        mesh = D.Mesh.box_tets(6)
+        p = D.params_box(0.05)
+        p.Npinitial = 2
+        D.derive(mesh, p)
+        parts = D.fill_freestream(mesh, p, ppc=1.5, temperature=1.0, seed=seed)
     elif kind == "box_hex":
         mesh = D.Mesh.box_hexes(6)
         p = D.params_box(0.05)
@@ -308,7 +317,8 @@ def test_track_parity_fast_particles(kind):
                                            ("box_dense", None), ("box_dense", -1), ("capsule", 0),
                                            ("box_dense", 0), ("box_dense", 20), ("box_crowded", None),
                                            ("box_crowded", -1), ("box_packed", 0),
-                                           ("box_vhs", -1), ("box_vhs", 0), ("capsule_morton", None)])
+                                           ("box_vhs", -1), ("box_vhs", 0), ("capsule_morton", None),
+                                           ("box_sparse", None), ("box_sparse", 0)])
 def test_full_step_parity(kind, wide_min):
     mesh, p, parts = _case(kind)
     print(f"[full_step_parity] {kind} wide_min={wide_min} cells={mesh.ncell} particles={parts.n}", flush=True)
