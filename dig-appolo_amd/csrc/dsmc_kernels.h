@@ -84,6 +84,7 @@ struct Counters {
   int32_t park_count;  // particles parked at a wall by the move kernel this step
   int32_t wide_n;      // cells queued for the one-wave-per-cell kernel from the front (the largest)
   int32_t wide_back;   // cells queued for it from the back of the queue (the others)
+  int32_t arr_spill;   // a cell got more arrivals this regroup than its arrival list holds
   unsigned long long emitted, particle_steps;
   TallySlot tally[TALLY_SLOTS];
 };
