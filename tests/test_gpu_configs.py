@@ -95,7 +95,8 @@ def test_apollo_full_size_fp64_second_seed():
 @pytest.mark.timeout(300)
 def test_apollo_full_size_mixed_within_one_percent_of_fp64():
     """The mixed form's C_D and stagnation heat flux against FP64: within 1%
-    (north star), printed beside the FP64 seed-to-seed difference.  The
+    (the mixed form's accuracy target), printed beside the FP64 seed-to-seed
+    difference.  The
     stagnation heat flux is the mean over the 10% most windward wall faces
     (745 of 7,450); the mean over 2% is printed too."""
     if "fp64" not in _apollo or "fp64_b" not in _apollo:
